@@ -1,0 +1,216 @@
+"""Benchmark: spherical-harmonic round trips (spectral -> grid -> spectral) per second at
+T255 (lmax=257, mmax=256) on the octahedral Gaussian grid nlat_half=192, 16 layers x 4
+fields (K = 64 columns), split-precision MFMA Legendre vs the plain FP32 path.
+
+One step = one round trip of all 64 columns (sht_synthesis + sht_analysis) with inputs
+resident in HBM.  Prints ONE JSON line.  --gpus above 1 is not supported (one GPU).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
+FP32_PEAK = 157.3e12       # flop/s, MI355X FP32 (matrix)
+F16_PEAK = 2.5e15          # flop/s, MI355X dense FP16/BF16 MFMA
+KERNELS = ["legendre_synthesis", "fft_synthesis", "fft_analysis", "legendre_analysis"]
+
+
+def geometry(sw, lmax, mmax, n):
+    grid = sw.OctahedralGaussianGrid(n)
+    nlon = np.array([grid.nlon_ring(j) for j in range(n)])
+    mt = np.minimum(nlon // 2, mmax - 1)
+    # M = sum_j sum_{m <= mt_j} (lmax - m): complex x real MACs per column and direction
+    M = int(sum(sum(lmax - m for m in range(int(t) + 1)) for t in mt))
+    nfreq_used = int((mt + 1).sum())
+    nharm = sw.Spectrum(lmax, mmax).nharm
+    return grid, nlon, mt, M, nfreq_used, nharm
+
+
+def kernel_cost(name, K, M, nharm, npoints, nfreq_used, nlon, split):
+    """Algorithmic (bytes, flops, peak flop/s) of one launch of each kernel."""
+    fb = nfreq_used * K * 16  # Fourier rows handed between stages (float4 per column)
+    peak = F16_PEAK / 3 if split else FP32_PEAK  # 3 MFMA products per algorithmic product
+    if name.startswith("legendre"):
+        lam = M * 4  # Λ of the contributing rings, once per launch (FP32 or hi+lo 16-bit)
+        byts = nharm * K * 8 + lam + fb
+        flops = 4.0 * K * M
+        return byts, flops, peak
+    fft_flops = float(sum(5.0 * N * math.log2(N) for N in nlon)) * K  # N and S rings in one complex FFT
+    return npoints * K * 4 + fb, fft_flops, FP32_PEAK
+
+
+def run_mode(sw, torch, precision, spectrum, grid, coeffs_host, steps, warmup):
+    K = coeffs_host.shape[0]
+    S = sw.SpectralTransform(spectrum, grid, nlayers=K, precision=precision)
+    coeffs = sw.LowerTriangularArray(torch.from_numpy(coeffs_host).to(torch.complex64).cuda(), spectrum)
+    field = sw.Field(torch.empty((K, grid.npoints), device="cuda"), grid)
+    back = sw.LowerTriangularArray(torch.empty((K, spectrum.nharm), dtype=torch.complex64, device="cuda"), spectrum)
+    for _ in range(warmup):
+        sw.transform_(field, coeffs, S)
+        sw.transform_(back, field, S)
+    torch.cuda.synchronize()
+    L = sw.lib()
+    assert L.sht_profile_begin(S._plan, steps) == 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sw.transform_(field, coeffs, S)
+        sw.transform_(back, field, S)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    avg = (C.c_double * 4)()
+    cnt = (C.c_int32 * 4)()
+    assert L.sht_profile_end(S._plan, avg, cnt) == 0
+    # plain timing without profiling events (the throughput that is reported)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sw.transform_(field, coeffs, S)
+        sw.transform_(back, field, S)
+    torch.cuda.synchronize()
+    dt_plain = time.perf_counter() - t0
+    out = back.data.cpu().numpy().astype(np.complex128)
+    S.close()
+    return {"s_per_step": dt_plain / steps, "s_per_step_profiled": dt / steps,
+            "kernel_ms": {KERNELS[i]: avg[i] for i in range(4)}, "roundtrip": out}
+
+
+def cpu_baseline(spectrum_o, grid_o, coeffs, seconds=10.0):
+    """The oracle's FP32 C restatement of the reference CPU path on the host cores, timed on
+    whole round trips of the same workload until `seconds` have elapsed (at least one)."""
+    from oracle.cref import CRefTransform, lib as oracle_lib
+    from oracle.sht_oracle import OracleTransform
+    o = OracleTransform(spectrum_o, grid_o)
+    c = CRefTransform(o)
+    threads = min(16, os.cpu_count() or 1)
+    a = coeffs.astype(np.complex64)
+    c.analysis(c.synthesis(a, np.float32, nthreads=threads), np.float32, nthreads=threads)  # warm
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        f = c.synthesis(a, np.float32, nthreads=threads)
+        c.analysis(f, np.float32, nthreads=threads)
+        n += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "round-trips/s", "cores": threads, "kind": "port",
+            "sample": f"{n} full T255 K={coeffs.shape[0]} FP32 round trips ({dt:.1f} s, OpenMP {threads} threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", default="fp16x3", choices=["fp16x3", "bf16x3", "fp32"])
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the transform shards over columns without communication, see the design notes",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import __graft_entry__
+    from oracle.sht_oracle import Grid as OGrid, OracleTransform, Spectrum as OSpectrum
+    sw = __graft_entry__.load_package()
+    torch.cuda.set_device(0)
+
+    lmax, mmax, n, layers, fields = 257, 256, 192, 16, 4
+    K = layers * fields
+    spectrum = sw.Spectrum(lmax, mmax)
+    grid, nlon, mt, M, nfreq_used, nharm = geometry(sw, lmax, mmax, n)
+    coeffs = sw.seeded_spectrum(spectrum, K, seed=0)
+    coeffs = coeffs.astype(np.complex64).astype(np.complex128)  # the FP32 values the GPU sees
+
+    res = {p: run_mode(sw, torch, p, spectrum, grid, coeffs, args.steps, args.warmup)
+           for p in dict.fromkeys([args.precision, "fp32"])}
+    head = res[args.precision]
+
+    # accuracy against the FP64 oracle of the same algorithm (per field and degree band)
+    o = OracleTransform(OSpectrum(lmax, mmax), OGrid("octahedral", n))
+    ref = o.analysis(o.synthesis(coeffs))
+    bands = [(0, 15), (16, 63), (64, 127), (128, lmax - 1)]
+    errs = {}
+    for p, r in res.items():
+        worst = 0.0
+        per = {}
+        for f in range(fields):
+            cols = slice(f * layers, (f + 1) * layers)
+            for lo, hi in bands:
+                sel = (o.spectrum.l_of >= lo) & (o.spectrum.l_of <= hi)
+                e = float(np.linalg.norm(r["roundtrip"][cols][:, sel] - ref[cols][:, sel]) /
+                          np.linalg.norm(ref[cols][:, sel]))
+                per[f"f{f}_l{lo}-{hi}"] = e
+                worst = max(worst, e)
+        errs[p] = {"worst": worst, "per_field_band": per}
+
+    split = args.precision != "fp32"
+    costs = {k: kernel_cost(k, K, M, nharm, grid.npoints, nfreq_used, nlon, split and k.startswith("legendre"))
+             for k in KERNELS}
+    kms = head["kernel_ms"]
+    dominant = max(KERNELS, key=lambda k: kms[k])
+    byts, flops, peak = costs[dominant]
+    t = kms[dominant] * 1e-3
+    if byts / HBM_PEAK >= flops / peak:
+        roof = {"bound": "hbm", "achieved": byts / t / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s"}
+    else:
+        roof = {"bound": "mfma", "achieved": flops / t / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["kernel"] = dominant
+    roof["kernel_ms"] = kms[dominant]
+
+    # whole round trip against its compulsory bytes / flops
+    B_rt = 2 * nharm * K * 8 + 2 * grid.npoints * K * 4 + 2 * nharm * n * 4
+    F_rt = 2 * 4.0 * K * M
+    t_roof = max(B_rt / HBM_PEAK, F_rt / (F16_PEAK / 3 if split else FP32_PEAK))
+
+    value = 1.0 / head["s_per_step"]
+    out = {
+        "metric": "SH round-trips/sec at T255 L16x4 fields, split-prec MFMA vs FP32 @1e-5 rel",
+        "value": value,
+        "unit": "round-trips/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["s_per_step"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp16x3": "f16x3-f32acc", "bf16x3": "bf16x3-f32acc", "fp32": "f32"}[args.precision],
+        "data": "synthetic (seeded N(0,1)(l+1)^-1.5 spectrum)",
+        "config": {"workload": "T255 spectral->grid->spectral round trip, OctahedralGaussianGrid nlat_half=192, "
+                               "16 layers x 4 fields (K=64)", "lmax": lmax, "mmax": mmax, "nlat_half": n, "K": K,
+                   "precision": args.precision},
+        "roofline": roof,
+        "fp32_round_trips_per_s": 1.0 / res["fp32"]["s_per_step"],
+        "speedup_vs_fp32": res["fp32"]["s_per_step"] / head["s_per_step"],
+        "kernel_ms": {p: r["kernel_ms"] for p, r in res.items()},
+        "roundtrip_roofline_frac": t_roof / head["s_per_step"],
+        "max_rel_err_vs_fp64": {p: e["worst"] for p, e in errs.items()},
+        "rel_err_per_field_band": {p: e["per_field_band"] for p, e in errs.items()},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(OSpectrum(lmax, mmax), OGrid("octahedral", n), coeffs, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

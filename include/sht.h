@@ -1,0 +1,138 @@
+/* libsht: spherical harmonic transform engine for SpeedyWeather on AMD MI355X (gfx950).
+ *
+ * C ABI (plain pointers and sizes, no C++ or torch types) for the hot path of
+ * SpeedyTransforms' transform!:
+ *
+ *   spectral -> grid  transform!(field, coeffs, S; unscale_coslat)
+ *                     SpeedyTransforms/src/spectral_transform.jl:515-570
+ *   grid -> spectral  transform!(coeffs, field, S)
+ *                     SpeedyTransforms/src/spectral_transform.jl:579-631
+ *
+ * A host binding (Julia ccall, Python ctypes) overrides the positional cores
+ * _transform_grid! / _transform_spec! (spectral_transform.jl:533,597) with
+ * sht_synthesis / sht_analysis; a plan replaces the device-side state of one
+ * SpectralTransform (spectral_transform.jl:118-302).
+ *
+ * Conventions (same as the reference):
+ *   - spectral data: LowerTriangularArray layout (LowerTriangularArrays/src/
+ *     lower_triangular_array.jl:367): column m (0-based) of lmax-m degrees, m-major,
+ *     complex float interleaved; column k of K starts at coeffs + k*ld_spec (complex
+ *     elements).
+ *   - grid data: RingGrids Field layout (RingGrids/src/field.jl:7-18): rings north to
+ *     south, each from 0 degrees east; column k starts at field + k*ld_grid (floats).
+ *   - all data pointers passed to sht_synthesis / sht_analysis are DEVICE pointers.
+ *   - outputs are overwritten, inputs are not modified.
+ *
+ * Every function returns an int status (SHT_OK = 0); nothing throws across the ABI.
+ * sht_last_error() returns a message for the last failure on the calling thread.
+ */
+#ifndef SHT_H
+#define SHT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SHT_OK 0
+#define SHT_ERR_ARG 1          /* invalid argument (null pointer, bad enum) */
+#define SHT_ERR_DIM 2          /* size mismatch: DimensionMismatch(S, coeffs|field) */
+#define SHT_ERR_TYPE 3         /* number format mismatch */
+#define SHT_ERR_HIP 4          /* HIP runtime error */
+#define SHT_ERR_UNSUPPORTED 5  /* configuration the engine does not implement */
+#define SHT_ERR_ALLOC 6        /* out of device or host memory */
+
+/* grid kinds (RingGrids/src/grids/octahedral_gaussian.jl, full_gaussian.jl) */
+#define SHT_GRID_OCTAHEDRAL_GAUSSIAN 0
+#define SHT_GRID_FULL_GAUSSIAN 1
+
+/* Legendre shortcuts (SpeedyTransforms/src/legendre_shortcuts.jl:21-40):
+ * mmax_truncation(ring) = min(nlon / d, mmax - 1) with d = 2, 3, 4 */
+#define SHT_SHORTCUT_LINEAR 0
+#define SHT_SHORTCUT_QUADRATIC 1
+#define SHT_SHORTCUT_CUBIC 2
+
+/* arithmetic of the Legendre contraction (FFT and all other steps are FP32) */
+#define SHT_PREC_FP32 0      /* FP32 products, FP32 accumulation (MFMA f32) */
+#define SHT_PREC_FP16X3 1    /* scaled FP16 hi/lo split, 3 products, FP32 accumulation */
+#define SHT_PREC_BF16X3 2    /* BF16 hi/lo split, 3 products, FP32 accumulation */
+
+typedef struct sht_desc {
+    int32_t grid;        /* SHT_GRID_* */
+    int32_t nlat_half;   /* rings per hemisphere */
+    int32_t lmax;        /* 1-based number of degrees (Spectrum.lmax) */
+    int32_t mmax;        /* 1-based number of orders (Spectrum.mmax) */
+    int32_t nlayers;     /* capacity: largest K any call may pass (SpectralTransform.nlayers) */
+    int32_t shortcut;    /* SHT_SHORTCUT_* */
+    int32_t precision;   /* SHT_PREC_* */
+    int32_t device;      /* HIP device ordinal the plan lives on */
+} sht_desc;
+
+typedef struct sht_plan sht_plan;
+
+/* Sizes implied by a descriptor (host only, no GPU needed). */
+typedef struct sht_sizes {
+    int64_t npoints;     /* grid points per column */
+    int64_t nharm;       /* spectral coefficients per column */
+    int32_t nlat;        /* rings */
+    int32_t nlon_max;    /* points on the longest ring */
+    int64_t nfreq_total; /* sum over northern rings of nlon/2+1 (Fourier buffer rows) */
+    int64_t device_bytes;/* device memory the plan allocates */
+} sht_sizes;
+
+/* Precompute a plan (SpectralTransform(spectrum, grid; NF=Float32, nlayers,
+ * LegendreShortcut), spectral_transform.jl:118-302): Gaussian latitudes and weights and the
+ * Legendre polynomials in FP64, stored in the layout the precision mode needs; FFT
+ * factorisations and twiddles; scratch for nlayers columns.  Blocking, host side. */
+int sht_plan_create(const sht_desc *desc, sht_plan **plan);
+int sht_plan_destroy(sht_plan *plan);
+int sht_plan_sizes(const sht_desc *desc, sht_sizes *sizes);
+
+/* spectral -> grid, replaces _transform_grid!(field, coeffs, scratch, S, unscale_coslat)
+ * (spectral_transform.jl:533-570).  coeffs: K columns of nharm complex floats (stride
+ * ld_spec >= nharm); field: K columns of npoints floats (stride ld_grid >= npoints).
+ * Asynchronous on `stream` (a hipStream_t; 0 = null stream). */
+int sht_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *field,
+                  int64_t ld_grid, int32_t K, int32_t unscale_coslat, void *stream);
+
+/* grid -> spectral, replaces _transform_spec!(coeffs, field, scratch, S)
+ * (spectral_transform.jl:597-631). */
+int sht_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void *coeffs,
+                 int64_t ld_spec, int32_t K, void *stream);
+
+/* U, V = (u, v) cos(lat) spectra from vorticity and divergence, replaces
+ * UV_from_vordiv!(U, V, vor, div, S; radius) (SpeedyTransforms/src/spectral_gradients.jl:
+ * 309-366).  All four arrays: K columns of nharm complex floats with stride ld. */
+int sht_uv_from_vordiv(sht_plan *plan, const void *vor, const void *div, void *U, void *V,
+                       int64_t ld, int32_t K, double radius, void *stream);
+
+/* Per-kernel timing (no reference counterpart; the reference profiled ad hoc with
+ * CUDA.@profile / BenchmarkTools, docs/dev/2026-08/legendre-gpu-optimization.md).
+ * Between begin and end, the first max_calls launches of every kernel are bracketed by
+ * HIP events on the call's stream; end waits for them and returns the mean duration (ms)
+ * and launch count per kernel, indexed by SHT_KERNEL_*. */
+#define SHT_KERNEL_LEGENDRE_SYNTHESIS 0
+#define SHT_KERNEL_FFT_SYNTHESIS 1
+#define SHT_KERNEL_FFT_ANALYSIS 2
+#define SHT_KERNEL_LEGENDRE_ANALYSIS 3
+#define SHT_NKERNELS 4
+int sht_profile_begin(sht_plan *plan, int32_t max_calls);
+int sht_profile_end(sht_plan *plan, double *avg_ms, int32_t *counts);
+
+/* Host-only precompute exposed for verification (no GPU needed). */
+/* Gauss-Legendre nodes (ascending) and weights of degree nlat
+ * (RingGrids/src/grids/full_gaussian.jl:48-50, quadrature_weights.jl:11). */
+int sht_gauss_legendre(int32_t nlat, double *nodes, double *weights);
+/* λ_l^m(x) for all 0 <= m < mmax, m <= l < lmax in LowerTriangularArray order
+ * (the Λ column that spectral_transform.jl:167-172 stores per ring). */
+int sht_legendre_column(int32_t lmax, int32_t mmax, double x, double *out);
+
+const char *sht_last_error(void);
+const char *sht_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,316 @@
+"""FP64 CPU restatement of SpeedyTransforms' spherical harmonic transform.
+
+TEST INFRASTRUCTURE ONLY. This module is the checker the HIP engine is compared
+against; only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import it. The product (``speedyweather.jl_amd``) never imports or
+calls anything under ``oracle/``.
+
+It restates, loop structure for loop structure, the reference CPU path
+(all citations relative to the SpeedyWeather.jl repository root):
+
+* sizes: ``SpeedyTransforms/src/aliasing.jl:9-59`` (get_nlat_half, roundup_fft),
+  ``RingGrids/src/grids/octahedral_gaussian.jl:50-116`` (16+4j points per ring,
+  north-to-south ring order, mirrored south), ``RingGrids/src/grids/full_gaussian.jl:43-61``;
+* spectral packing: ``LowerTriangularArrays/src/lower_triangular_array.jl:367-374``
+  (m-major columns, l within a column);
+* Gaussian latitudes and weights: ``RingGrids/src/grids/full_gaussian.jl:48-50``,
+  ``RingGrids/src/quadrature_weights.jl:11,48-49`` (solid angles w_j * 2pi/nlon_j);
+* Legendre polynomials: ``SpeedyTransforms/src/spectral_transform.jl:167-172`` calling
+  ``Legendre.λlm!`` of AssociatedLegendrePolynomials.jl v1.0.2 (not vendored): the
+  spherical-harmonic normalised P_l^m with the Condon-Shortley phase, restated here by
+  the textbook three-term recursion in FP64;
+* Legendre shortcut: ``SpeedyTransforms/src/legendre_shortcuts.jl:21-25`` (linear);
+* inverse Legendre: ``SpeedyTransforms/src/legendre.jl:2-33,38-101``;
+* forward Legendre: ``SpeedyTransforms/src/legendre.jl:104-126,131-198``;
+* Fourier stage: ``SpeedyTransforms/src/fourier.jl:27-77,129-250`` with FFTW's
+  unnormalised rfft / brfft (FFTW.jl 1.10.0, not vendored), restated with numpy.fft
+  (rfft; irfft*N, which like brfft ignores Im of the DC and Nyquist bins);
+* vector path: ``SpeedyTransforms/src/spectral_gradients.jl:309-366`` with the factors
+  of ``SpeedyTransforms/src/gradient_arrays.jl:4-21,98-145``.
+
+Pinning (tests/test_oracle.py): Gauss nodes/weights against numpy's leggauss and exact
+polynomial quadrature; Λ against closed forms for l <= 2, against scipy's spherical
+harmonics, discrete orthonormality and the recursion residual; the FFT edge semantics
+against a brute-force DFT; the whole transform against Y00 -> constant grid, single
+harmonic round trips, the FP64 grid->spec->grid round trip at cubic dealiasing, the
+vector path against solid-body rotation. Parity unpinned: the Condon-Shortley sign of
+odd-m polynomials inside AssociatedLegendrePolynomials.jl (no reference test fixes it).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+# ----------------------------------------------------------------------------- sizes
+
+
+def roundup_fft(n: int, small_primes=(2, 3, 5)) -> int:
+    """Smallest even integer >= n whose prime factors are all in small_primes
+    (SpeedyTransforms/src/aliasing.jl:42-59)."""
+    n += n % 2
+    while True:
+        r = n
+        for p in small_primes:
+            while r % p == 0:
+                r //= p
+        if r == 1:
+            return n
+        n += 2
+
+
+def get_nlat_half(truncation: int, dealiasing: float = 2.0) -> int:
+    """aliasing.jl:9-15, truncation 1-based (= mmax of the Spectrum)."""
+    return roundup_fft(math.ceil(((1 + dealiasing) * (truncation - 1) + 1) / 4))
+
+
+def get_truncation(nlat_half: int, dealiasing: float = 2.0) -> int:
+    """aliasing.jl:23-29."""
+    return math.floor((4 * nlat_half - 1) / (dealiasing + 1)) + 1
+
+
+class Spectrum:
+    """1-based (lmax, mmax) as LowerTriangularArrays/src/spectrum.jl:6-70."""
+
+    def __init__(self, lmax: int, mmax: int):
+        assert lmax >= mmax >= 1
+        self.lmax, self.mmax = lmax, mmax
+        self.nharm = lmax * mmax - mmax * (mmax - 1) // 2
+        m = np.arange(mmax)
+        self.offsets = m * lmax - m * (m - 1) // 2  # 0-based start of column m
+        ls, ms = [], []
+        for mm in range(mmax):
+            for ll in range(mm, lmax):
+                ls.append(ll)
+                ms.append(mm)
+        self.l_of = np.array(ls)
+        self.m_of = np.array(ms)
+
+    def index(self, l: int, m: int) -> int:
+        """0-based packed index of degree l, order m (lower_triangular_array.jl:367)."""
+        return int(self.offsets[m] + (l - m))
+
+
+class Grid:
+    """Octahedral or full Gaussian ring grid (octahedral_gaussian.jl:50-116,
+    full_gaussian.jl:43-61): nlat = 2 nlat_half rings north to south, south mirrored."""
+
+    def __init__(self, kind: str, nlat_half: int):
+        assert kind in ("octahedral", "full")
+        self.kind, self.nlat_half = kind, nlat_half
+        n = nlat_half
+        if kind == "octahedral":
+            north = [16 + 4 * j for j in range(1, n + 1)]
+        else:
+            north = [4 * n] * n
+        self.nlons_half = np.array(north)
+        self.nlons = np.array(north + north[::-1])
+        self.ring_start = np.concatenate([[0], np.cumsum(self.nlons)[:-1]])
+        self.npoints = int(self.nlons.sum())
+        self.nlat = 2 * n
+
+
+# ------------------------------------------------------------------ quadrature, Λ
+
+
+def _legendre_p_and_dp(N: int, x: np.ndarray):
+    p0 = np.ones_like(x)
+    p1 = x.copy()
+    for k in range(2, N + 1):
+        p0, p1 = p1, ((2 * k - 1) * x * p1 - (k - 1) * p0) / k
+    dp = N * (x * p1 - p0) / (x * x - 1)
+    return p1, dp
+
+
+def gauss_legendre(N: int):
+    """Nodes (ascending) and weights of N-point Gauss-Legendre quadrature, Newton
+    iteration on P_N in FP64 (FastGaussQuadrature.gausslegendre's contract,
+    RingGrids/src/grids/full_gaussian.jl:49)."""
+    i = np.arange(1, N + 1)
+    x = -np.cos(np.pi * (i - 0.25) / (N + 0.5))
+    for _ in range(100):
+        p, dp = _legendre_p_and_dp(N, x)
+        dx = p / dp
+        x = x - dx
+        if np.max(np.abs(dx)) < 1e-16:
+            break
+    p, dp = _legendre_p_and_dp(N, x)
+    w = 2.0 / ((1 - x * x) * dp * dp)
+    return x, w
+
+
+def legendre_lambda(lmax: int, mmax: int, x) -> np.ndarray:
+    """λ_l^m(x) for 0 <= m < mmax, m <= l < lmax, packed as a LowerTriangularArray
+    column (m-major); x scalar or 1-D array (then one column per x). Orthonormal on the
+    sphere (λ_0^0 = 1/(2 sqrt(pi))), Condon-Shortley phase (-1)^m. Three-term recursion
+    in l from the diagonal λ_m^m, all in FP64."""
+    xa = np.atleast_1d(np.asarray(x, dtype=np.float64))
+    s = np.sqrt(np.maximum(0.0, 1.0 - xa * xa))
+    out = np.zeros((lmax * mmax - mmax * (mmax - 1) // 2, xa.size))
+    pmm = np.full(xa.size, 1.0 / math.sqrt(4 * math.pi))
+    idx = 0
+    for m in range(mmax):
+        if m > 0:
+            pmm = -math.sqrt((2 * m + 1) / (2.0 * m)) * s * pmm
+        out[idx] = pmm
+        if m + 1 < lmax:
+            p0 = pmm
+            p1 = math.sqrt(2 * m + 3) * xa * pmm
+            out[idx + 1] = p1
+            for l in range(m + 2, lmax):
+                a = math.sqrt((4.0 * l * l - 1) / (l * l - m * m))
+                b = math.sqrt(((l - 1.0) ** 2 - m * m) / (4.0 * (l - 1) ** 2 - 1))
+                p0, p1 = p1, a * (xa * p1 - b * p0)
+                out[idx + l - m] = p1
+        idx += lmax - m
+    return out[:, 0] if np.ndim(x) == 0 else out
+
+
+class OracleTransform:
+    """FP64 restatement of SpectralTransform + transform! (spectral_transform.jl:118-302,
+    515-631) for octahedral / full Gaussian grids with the linear Legendre shortcut."""
+
+    def __init__(self, spectrum: Spectrum, grid: Grid, shortcut: str = "linear"):
+        self.spectrum, self.grid = spectrum, grid
+        n = grid.nlat_half
+        x, w = gauss_legendre(2 * n)
+        latd = np.degrees(np.arccos(x)) - 90.0  # full_gaussian.jl:49 (north first)
+        self.latd = latd
+        colat = np.radians(90.0 - latd)
+        self.cos_colat = np.cos(colat)  # spectral_transform.jl:147-149
+        self.coslat = np.cos(np.radians(latd))  # :150
+        self.coslat_inv = 1.0 / self.coslat
+        self.weights = w
+        self.solid_angles = w * 2 * np.pi / grid.nlons  # quadrature_weights.jl:48-49
+        div = {"linear": 2, "quadratic": 3, "cubic": 4}[shortcut]
+        self.mtrunc = np.minimum(grid.nlons_half // div, spectrum.mmax - 1)  # :154-155
+        self.nfreq = grid.nlons_half // 2 + 1
+        # Λ.data[lm, j] for the northern rings (spectral_transform.jl:167-172)
+        self.legendre = legendre_lambda(spectrum.lmax, spectrum.mmax, self.cos_colat[:n])
+        self.norm_sphere = 2 * math.sqrt(math.pi)
+
+    # -- spectral -> grid (legendre.jl:38-101, fourier.jl:215-250)
+    def inverse_legendre(self, coeffs: np.ndarray, unscale_coslat: bool = False):
+        """coeffs (K, nharm) complex -> g_north, g_south (n, K, nfreq_max) complex."""
+        sp, n = self.spectrum, self.grid.nlat_half
+        K = coeffs.shape[0]
+        nfmax = int(self.nfreq.max())
+        gN = np.zeros((n, K, nfmax), complex)
+        gS = np.zeros((n, K, nfmax), complex)
+        for m in range(sp.mmax):
+            o = sp.offsets[m]
+            Lm = sp.lmax - m
+            rings = np.nonzero(self.mtrunc >= m)[0]
+            if rings.size == 0:
+                continue
+            lam = self.legendre[o:o + Lm][:, rings]  # (Lm, J)
+            a = coeffs[:, o:o + Lm]  # (K, Lm)
+            even = a[:, 0::2] @ lam[0::2]  # (K, J)
+            odd = a[:, 1::2] @ lam[1::2]
+            gN[rings, :, m] = (even + odd).T
+            gS[rings, :, m] = (even - odd).T
+        if unscale_coslat:
+            gN *= self.coslat_inv[:n, None, None]
+            gS *= self.coslat_inv[:n, None, None]
+        return gN, gS
+
+    def inverse_fourier(self, gN, gS):
+        g = self.grid
+        n = g.nlat_half
+        K = gN.shape[1]
+        field = np.zeros((K, g.npoints))
+        for j in range(n):
+            N = int(g.nlons_half[j])
+            nf = N // 2 + 1
+            js = g.nlat - 1 - j
+            field[:, g.ring_start[j]:g.ring_start[j] + N] = np.fft.irfft(gN[j, :, :nf], n=N) * N
+            field[:, g.ring_start[js]:g.ring_start[js] + N] = np.fft.irfft(gS[j, :, :nf], n=N) * N
+        return field
+
+    def synthesis(self, coeffs: np.ndarray, unscale_coslat: bool = False) -> np.ndarray:
+        coeffs = np.atleast_2d(coeffs)
+        return self.inverse_fourier(*self.inverse_legendre(coeffs, unscale_coslat))
+
+    # -- grid -> spectral (fourier.jl:129-166, legendre.jl:131-198)
+    def forward_fourier(self, field: np.ndarray):
+        g = self.grid
+        n = g.nlat_half
+        K = field.shape[0]
+        nfmax = int(self.nfreq.max())
+        fN = np.zeros((n, K, nfmax), complex)
+        fS = np.zeros((n, K, nfmax), complex)
+        for j in range(n):
+            N = int(g.nlons_half[j])
+            nf = N // 2 + 1
+            js = g.nlat - 1 - j
+            fN[j, :, :nf] = np.fft.rfft(field[:, g.ring_start[j]:g.ring_start[j] + N])
+            fS[j, :, :nf] = np.fft.rfft(field[:, g.ring_start[js]:g.ring_start[js] + N])
+        return fN, fS
+
+    def forward_legendre(self, fN, fS):
+        sp, n = self.spectrum, self.grid.nlat_half
+        K = fN.shape[1]
+        coeffs = np.zeros((K, sp.nharm), complex)
+        dO = self.solid_angles[:n]
+        for m in range(sp.mmax):
+            rings = np.nonzero(self.mtrunc >= m)[0]
+            if rings.size == 0:
+                continue
+            o = sp.offsets[m]
+            Lm = sp.lmax - m
+            lam = self.legendre[o:o + Lm][:, rings]  # (Lm, J)
+            even = dO[rings, None] * (fN[rings, :, m] + fS[rings, :, m])  # (J, K)
+            odd = dO[rings, None] * (fN[rings, :, m] - fS[rings, :, m])
+            out = np.zeros((Lm, K), complex)
+            out[0::2] = lam[0::2] @ even
+            out[1::2] = lam[1::2] @ odd
+            coeffs[:, o:o + Lm] = out.T
+        return coeffs
+
+    def analysis(self, field: np.ndarray) -> np.ndarray:
+        field = np.atleast_2d(field)
+        return self.forward_legendre(*self.forward_fourier(field))
+
+
+# --------------------------------------------------------------- vector path (C2)
+
+
+def uv_from_vordiv(spectrum: Spectrum, vor: np.ndarray, div: np.ndarray, radius: float = 1.0):
+    """U, V = (u, v) cos(lat) spectra from vorticity and divergence
+    (spectral_gradients.jl:309-366; factors gradient_arrays.jl:4-21,98-145).
+    vor, div: (K, nharm) complex."""
+    lmax, mmax = spectrum.lmax, spectrum.mmax
+
+    def eps(l, m):
+        return math.sqrt((l * l - m * m) / (4.0 * l * l - 1)) if l > 0 else 0.0
+
+    vor = np.atleast_2d(vor)
+    div = np.atleast_2d(div)
+    U = np.zeros_like(vor)
+    V = np.zeros_like(vor)
+    for m in range(mmax):
+        o = spectrum.offsets[m]
+        for l in range(m, lmax):
+            i = o + (l - m)
+            x = -m / (l * (l + 1)) if l > 0 else 0.0
+            u1 = eps(l, m) / l if l > 0 else 0.0
+            u2 = eps(l + 1, m) / (l + 1) if l < lmax - 1 else 0.0
+            z = 1j * x
+            first = (i == 0)
+            if first:
+                U[:, i] = z * div[:, i] + u2 * vor[:, i + 1]
+                V[:, i] = z * vor[:, i] - u2 * div[:, i + 1]
+            elif l == lmax - 2:
+                U[:, i] = z * div[:, i] - u1 * vor[:, i - 1]
+                V[:, i] = z * vor[:, i] + u1 * div[:, i - 1]
+            elif l == lmax - 1:
+                U[:, i] = -u1 * vor[:, i - 1]
+                V[:, i] = u1 * div[:, i - 1]
+            else:
+                up = vor[:, i + 1]
+                dp = div[:, i + 1]
+                vm = vor[:, i - 1] if l > m else 0.0
+                dm = div[:, i - 1] if l > m else 0.0
+                U[:, i] = z * div[:, i] + u2 * up - u1 * vm
+                V[:, i] = z * vor[:, i] + u1 * dm - u2 * dp
+    return U * radius, V * radius

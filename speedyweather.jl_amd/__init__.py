@@ -1,0 +1,14 @@
+"""MI355X spherical harmonic transform engine for SpeedyWeather (host-side mirror of
+SpeedyTransforms' API over libsht's C ABI; hand-written gfx950 kernels underneath)."""
+from ._lib import DimensionMismatch, SHTError, lib
+from .grids import (Field, FullGaussianGrid, OctahedralGaussianGrid, get_nlat_half, get_truncation,
+                    roundup_fft)
+from .spectrum import LowerTriangularArray, Spectrum
+from .synthetic import seeded_grid, seeded_spectrum
+from .transform import SpectralTransform, UV_from_vordiv_, transform, transform_
+
+__all__ = [
+    "DimensionMismatch", "SHTError", "lib", "Field", "FullGaussianGrid", "OctahedralGaussianGrid",
+    "get_nlat_half", "get_truncation", "roundup_fft", "LowerTriangularArray", "Spectrum",
+    "seeded_grid", "seeded_spectrum", "SpectralTransform", "UV_from_vordiv_", "transform", "transform_",
+]

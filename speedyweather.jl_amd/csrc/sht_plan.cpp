@@ -1,0 +1,735 @@
+// Host side of libsht: plan construction (FP64 precompute), validation and the C ABI.
+//
+// The precompute follows SpectralTransform (SpeedyTransforms/src/spectral_transform.jl:
+// 118-302): ring sizes and order (RingGrids/src/grids/octahedral_gaussian.jl:50-116,
+// full_gaussian.jl:43-61), Gaussian latitudes and weights (full_gaussian.jl:48-50,
+// quadrature_weights.jl:11,48-49), Legendre shortcut (legendre_shortcuts.jl:21-40),
+// Legendre polynomials (spectral_transform.jl:167-172) computed here in FP64 and stored
+// in the order the gfx950 kernels read them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/sht.h"
+#include "sht_internal.h"
+
+using sht::DevPlan;
+using sht::LegTile;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+// ------------------------------------------------------------------ geometry
+
+struct Geometry {
+    int n = 0, nlat = 0, lmax = 0, mmax = 0, nharm = 0, nlon_max = 0;
+    long long npoints = 0, fb_rows = 0;
+    std::vector<int> nlon, mt, j0;
+    std::vector<long long> ring_n, ring_s, fboff, lm_off;
+};
+
+int make_geometry(const sht_desc *d, Geometry &g) {
+    if (!d) return fail(SHT_ERR_ARG, "null descriptor");
+    if (d->grid != SHT_GRID_OCTAHEDRAL_GAUSSIAN && d->grid != SHT_GRID_FULL_GAUSSIAN)
+        return fail(SHT_ERR_UNSUPPORTED, "grid kind must be octahedral or full Gaussian");
+    if (d->nlat_half < 1 || d->nlat_half > 4096) return fail(SHT_ERR_ARG, "nlat_half out of range");
+    if (d->mmax < 1 || d->lmax < d->mmax || d->lmax > 8192)
+        return fail(SHT_ERR_ARG, "need 1 <= mmax <= lmax <= 8192");
+    if (d->nlayers < 1) return fail(SHT_ERR_ARG, "nlayers must be >= 1");
+    if (d->shortcut < SHT_SHORTCUT_LINEAR || d->shortcut > SHT_SHORTCUT_CUBIC)
+        return fail(SHT_ERR_UNSUPPORTED, "unknown Legendre shortcut");
+    if (d->precision < SHT_PREC_FP32 || d->precision > SHT_PREC_BF16X3)
+        return fail(SHT_ERR_ARG, "unknown precision mode");
+    const int n = d->nlat_half;
+    g.n = n;
+    g.nlat = 2 * n;
+    g.lmax = d->lmax;
+    g.mmax = d->mmax;
+    g.nharm = d->lmax * d->mmax - d->mmax * (d->mmax - 1) / 2;
+    g.nlon.resize(n);
+    for (int j = 0; j < n; ++j)
+        g.nlon[j] = d->grid == SHT_GRID_OCTAHEDRAL_GAUSSIAN ? 16 + 4 * (j + 1) : 4 * n;
+    g.nlon_max = *std::max_element(g.nlon.begin(), g.nlon.end());
+    // ring ranges: northern rings 0..n-1, then southern rings mirrored
+    g.ring_n.resize(n);
+    g.ring_s.resize(n);
+    long long pos = 0;
+    for (int j = 0; j < n; ++j) { g.ring_n[j] = pos; pos += g.nlon[j]; }
+    for (int j = n - 1; j >= 0; --j) { g.ring_s[j] = pos; pos += g.nlon[j]; }
+    g.npoints = pos;
+    // Legendre shortcut: mt_j = min(nlon_j / div, mmax - 1)
+    const int div = d->shortcut == SHT_SHORTCUT_LINEAR ? 2 : (d->shortcut == SHT_SHORTCUT_QUADRATIC ? 3 : 4);
+    g.mt.resize(n);
+    for (int j = 0; j < n; ++j) g.mt[j] = std::min(g.nlon[j] / div, g.mmax - 1);
+    for (int j = 1; j < n; ++j)
+        if (g.mt[j] < g.mt[j - 1]) return fail(SHT_ERR_UNSUPPORTED, "non-monotonic shortcut");
+    g.fboff.resize(n);
+    long long rows = 0;
+    for (int j = 0; j < n; ++j) { g.fboff[j] = rows; rows += g.nlon[j] / 2 + 1; }
+    g.fb_rows = rows;
+    g.j0.resize(g.mmax);
+    for (int m = 0; m < g.mmax; ++m) {
+        int j = 0;
+        while (j < n && g.mt[j] < m) ++j;
+        g.j0[m] = j;
+    }
+    g.lm_off.resize(g.mmax);
+    for (int m = 0; m < g.mmax; ++m) g.lm_off[m] = (long long)m * g.lmax - (long long)m * (m - 1) / 2;
+    return SHT_OK;
+}
+
+// ------------------------------------------------------------- quadrature, Λ
+
+void legendre_pn(int N, double x, double &p, double &dp) {
+    double p0 = 1.0, p1 = x;
+    for (int k = 2; k <= N; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+    }
+    p = N == 0 ? 1.0 : p1;
+    dp = N * (x * p1 - p0) / (x * x - 1.0);
+}
+
+// Gauss-Legendre nodes (ascending) and weights: Newton on P_N from the Tricomi-type
+// initial guess, weights 2 / ((1-x^2) P_N'(x)^2).
+void gauss_legendre(int N, std::vector<double> &x, std::vector<double> &w) {
+    x.resize(N);
+    w.resize(N);
+    for (int i = 0; i < (N + 1) / 2; ++i) {
+        double z = std::cos(M_PI * (i + 0.75) / (N + 0.5));  // i-th largest root
+        for (int it = 0; it < 100; ++it) {
+            double p, dp;
+            legendre_pn(N, z, p, dp);
+            const double dz = p / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-15) break;
+        }
+        double p, dp;
+        legendre_pn(N, z, p, dp);
+        z -= p / dp;  // one more step at full precision
+        legendre_pn(N, z, p, dp);
+        const double wi = 2.0 / ((1.0 - z * z) * dp * dp);
+        x[N - 1 - i] = z;
+        x[i] = -z;
+        w[i] = w[N - 1 - i] = wi;
+    }
+    if (N % 2 == 1) x[N / 2] = 0.0;
+}
+
+// λ_l^m(x) for one order m and l = m..lmax-1, from λ_m^m (given) by the three-term
+// recursion of the orthonormal associated Legendre functions.
+void legendre_order(int lmax, int m, double x, double pmm, double *out) {
+    out[0] = pmm;
+    if (m + 1 >= lmax) return;
+    double p0 = pmm, p1 = std::sqrt(2.0 * m + 3.0) * x * pmm;
+    out[1] = p1;
+    for (int l = m + 2; l < lmax; ++l) {
+        const double a = std::sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+        const double b = std::sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+        const double p2 = a * (x * p1 - b * p0);
+        p0 = p1;
+        p1 = p2;
+        out[l - m] = p2;
+    }
+}
+
+// λ_m^m from λ_{m-1}^{m-1}: Condon-Shortley phase, λ_0^0 = 1/sqrt(4 pi)
+inline double next_pmm(double pmm_prev, int m, double s) {
+    return m == 0 ? 1.0 / std::sqrt(4.0 * M_PI) : -std::sqrt((2.0 * m + 1.0) / (2.0 * m)) * s * pmm_prev;
+}
+
+// ---------------------------------------------------------- 16-bit conversions
+
+uint16_t to_half(double v) {
+    const _Float16 h = (_Float16)(float)v;
+    uint16_t u;
+    std::memcpy(&u, &h, 2);
+    return u;
+}
+double half_value(uint16_t u) {
+    _Float16 h;
+    std::memcpy(&h, &u, 2);
+    return (double)(float)h;
+}
+uint16_t to_bf16(double v) {
+    const float f = (float)v;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+double bf16_value(uint16_t b) {
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return (double)f;
+}
+
+// power of two mapping amax into [2^14, 2^15)
+double fp16_scale(double amax) {
+    if (!(amax > 0.0)) return 1.0;
+    int e;
+    std::frexp(amax, &e);
+    int sh = std::max(-120, std::min(120, 15 - e));
+    return std::ldexp(1.0, sh);
+}
+
+template <class T>
+int dev_upload(std::vector<void *> &allocs, const std::vector<T> &h, const T **out) {
+    *out = nullptr;
+    if (h.empty()) return SHT_OK;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, h.size() * sizeof(T));
+    if (e != hipSuccess) return fail(SHT_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocs.push_back(p);
+    e = hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    *out = static_cast<const T *>(p);
+    return SHT_OK;
+}
+
+}  // namespace
+
+// Optional per-kernel timing: events recorded around every launch on the call's stream.
+struct Profile {
+    bool on = false;
+    int max_calls = 0;
+    int ncalls[SHT_NKERNELS] = {0};
+    std::vector<hipEvent_t> ev;  // [(kernel * max_calls + call) * 2 + {start, stop}]
+};
+
+struct sht_plan {
+    sht_desc d{};
+    Geometry g;
+    DevPlan P{};
+    std::vector<void *> allocs;
+    const LegTile *syn_tiles = nullptr;
+    const LegTile *ana_tiles = nullptr;
+    int nsyn = 0, nana = 0;
+    void *fbuf = nullptr;
+    long long device_bytes = 0;
+    Profile prof;
+};
+
+namespace {
+// Brackets one kernel launch with profiling events when profiling is on.
+struct ProfScope {
+    sht_plan *pl;
+    int k;
+    hipStream_t s;
+    bool rec = false;
+    ProfScope(sht_plan *p, int kernel, void *stream) : pl(p), k(kernel), s(reinterpret_cast<hipStream_t>(stream)) {
+        Profile &pr = pl->prof;
+        if (pr.on && pr.ncalls[k] < pr.max_calls) {
+            rec = true;
+            (void)hipEventRecord(pr.ev[((size_t)k * pr.max_calls + pr.ncalls[k]) * 2], s);
+        }
+    }
+    ~ProfScope() {
+        Profile &pr = pl->prof;
+        if (rec) {
+            (void)hipEventRecord(pr.ev[((size_t)k * pr.max_calls + pr.ncalls[k]) * 2 + 1], s);
+            ++pr.ncalls[k];
+        }
+    }
+};
+}  // namespace
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    bool changed = false;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev) {
+            changed = hipSetDevice(dev) == hipSuccess;
+        }
+    }
+    ~DeviceGuard() {
+        if (changed) (void)hipSetDevice(prev);
+    }
+};
+
+void free_plan(sht_plan *p) {
+    if (!p) return;
+    for (hipEvent_t e : p->prof.ev) (void)hipEventDestroy(e);
+    for (void *a : p->allocs) (void)hipFree(a);
+    delete p;
+}
+
+int factorize(int N, int *fac) {
+    int nf = 0, r = N;
+    while (r % 4 == 0) { fac[nf++] = 4; r /= 4; }
+    const int small[3] = {2, 3, 5};
+    for (int s : small)
+        while (r % s == 0) { fac[nf++] = s; r /= s; }
+    for (int p = 7; r > 1; p += 2)
+        while (r % p == 0) { fac[nf++] = p; r /= p; }
+    return nf;
+}
+
+int build_plan(sht_plan *pl) {
+    const sht_desc &d = pl->d;
+    Geometry &g = pl->g;
+    const int n = g.n, lmax = g.lmax, mmax = g.mmax;
+    const int mode = d.precision;
+    std::vector<void *> &A = pl->allocs;
+    DevPlan &P = pl->P;
+    int rc;
+
+    // latitudes: nodes ascending, ring j (north first) has cos(colat) = -x_j
+    std::vector<double> x, w;
+    gauss_legendre(2 * n, x, w);
+    std::vector<double> cosc(n), sinc(n), coslat_inv(n), dOmega(n);
+    for (int j = 0; j < n; ++j) {
+        cosc[j] = -x[j];
+        sinc[j] = std::sqrt(std::max(0.0, 1.0 - x[j] * x[j]));
+        coslat_inv[j] = 1.0 / sinc[j];  // cos(lat) = sin(colat)
+        dOmega[j] = w[j] * 2.0 * M_PI / g.nlon[j];
+    }
+
+    // ---------------------------------------------------- Λ in kernel order
+    std::vector<long long> syn_off(mmax), ana_off(mmax), syn_soff(mmax), ana_soff(mmax);
+    long long syn_total = 0, ana_total = 0, syn_stotal = 0, ana_stotal = 0;
+    const bool split = mode != SHT_PREC_FP32;
+    for (int m = 0; m < mmax; ++m) {
+        const int J = n - g.j0[m], L = lmax - m, Le = (L + 1) / 2;
+        const int Jt = (J + 15) / 16, nch = (Le + sht::kChunk - 1) / sht::kChunk;
+        const int NG = (L + sht::kDegPerWG - 1) / sht::kDegPerWG, nt = 2 * NG;
+        const int JC = (J + sht::kChunk - 1) / sht::kChunk;
+        syn_off[m] = syn_total;
+        ana_off[m] = ana_total;
+        syn_soff[m] = syn_stotal;
+        ana_soff[m] = ana_stotal;
+        syn_total += (long long)2 * Jt * nch * sht::kChunk * 16;  // same count in both layouts
+        ana_total += (long long)2 * nt * JC * sht::kChunk * 16;
+        syn_stotal += (long long)Jt * 16;
+        ana_stotal += (long long)2 * nt * 16;
+    }
+    std::vector<float> syn_f32, ana_f32, syn_sinv, ana_sinv;
+    std::vector<uint16_t> syn_hi, syn_lo, ana_hi, ana_lo;
+    if (split) {
+        syn_hi.assign(syn_total, 0); syn_lo.assign(syn_total, 0);
+        ana_hi.assign(ana_total, 0); ana_lo.assign(ana_total, 0);
+        syn_sinv.assign(syn_stotal, 1.f); ana_sinv.assign(ana_stotal, 1.f);
+    } else {
+        syn_f32.assign(syn_total, 0.f);
+        ana_f32.assign(ana_total, 0.f);
+    }
+    auto split_store = [&](double v, double s, uint16_t &hi, uint16_t &lo) {
+        if (mode == SHT_PREC_FP16X3) {
+            const double xs = v * s;
+            hi = to_half(xs);
+            lo = to_half(xs - half_value(hi));
+        } else {
+            hi = to_bf16(v);
+            lo = to_bf16(v - bf16_value(hi));
+        }
+    };
+
+    std::vector<double> pmm(n, 0.0);
+    std::vector<double> lam;  // [j - j0][l - m]
+    for (int m = 0; m < mmax; ++m) {
+        for (int j = 0; j < n; ++j) pmm[j] = next_pmm(pmm[j], m, sinc[j]);
+        const int j0 = g.j0[m], J = n - j0, L = lmax - m, Le = (L + 1) / 2;
+        if (J == 0) continue;
+        lam.assign((size_t)J * L, 0.0);
+        for (int jj = 0; jj < J; ++jj) legendre_order(lmax, m, cosc[j0 + jj], pmm[j0 + jj], &lam[(size_t)jj * L]);
+        auto LAM = [&](int l, int j) -> double { return lam[(size_t)(j - j0) * L + (l - m)]; };
+        const int Jt = (J + 15) / 16, nch = (Le + sht::kChunk - 1) / sht::kChunk;
+        const int NG = (L + sht::kDegPerWG - 1) / sht::kDegPerWG, nt = 2 * NG;
+        const int JC = (J + sht::kChunk - 1) / sht::kChunk;
+        // synthesis: rows = rings, K = l' per parity
+        std::vector<double> srow_scale(Jt * 16, 1.0);
+        if (mode == SHT_PREC_FP16X3) {
+            for (int t = 0; t < Jt; ++t)
+                for (int r = 0; r < 16; ++r) {
+                    const int j = j0 + 16 * t + r;
+                    if (j >= n) continue;
+                    double amax = 0.0;
+                    for (int l = m; l < lmax; ++l) amax = std::max(amax, std::fabs(LAM(l, j)));
+                    const double s = fp16_scale(amax);
+                    srow_scale[t * 16 + r] = s;
+                    syn_sinv[syn_soff[m] + t * 16 + r] = (float)(1.0 / s);
+                }
+        }
+        for (int p = 0; p < 2; ++p)
+            for (int t = 0; t < Jt; ++t)
+                for (int lp = 0; lp < nch * sht::kChunk; ++lp) {
+                    const int l = m + 2 * lp + p;
+                    if (l >= lmax) continue;
+                    for (int r = 0; r < 16; ++r) {
+                        const int j = j0 + 16 * t + r;
+                        if (j >= n) continue;
+                        const double v = LAM(l, j);
+                        if (!split) {
+                            syn_f32[syn_off[m] + ((long long)(p * Jt + t) * (nch * sht::kChunk) + lp) * 16 + r] = (float)v;
+                        } else {
+                            const int c = lp / sht::kChunk, kq = lp % sht::kChunk;
+                            const int grp = kq / 8, e = kq % 8, lane = grp * 16 + r;
+                            const long long idx = syn_off[m] + (((long long)(p * Jt + t) * nch + c) * 64 + lane) * 8 + e;
+                            split_store(v, srow_scale[t * 16 + r], syn_hi[idx], syn_lo[idx]);
+                        }
+                    }
+                }
+        // analysis: rows = l' per parity, K = rings; values ΔΩ_j λ_l^m(x_j)
+        if (JC == 0) continue;
+        for (int p = 0; p < 2; ++p)
+            for (int lt = 0; lt < nt; ++lt)
+                for (int r = 0; r < 16; ++r) {
+                    const int lq = 16 * lt + r, l = m + 2 * lq + p;
+                    if (l >= lmax) continue;
+                    double s = 1.0;
+                    if (mode == SHT_PREC_FP16X3) {
+                        double amax = 0.0;
+                        for (int j = j0; j < n; ++j) amax = std::max(amax, std::fabs(dOmega[j] * LAM(l, j)));
+                        s = fp16_scale(amax);
+                        ana_sinv[ana_soff[m] + (p * nt + lt) * 16 + r] = (float)(1.0 / s);
+                    }
+                    for (int jp = 0; jp < J; ++jp) {
+                        const int j = j0 + jp;
+                        const double v = dOmega[j] * LAM(l, j);
+                        if (!split) {
+                            ana_f32[ana_off[m] + ((long long)(p * nt + lt) * (JC * sht::kChunk) + jp) * 16 + r] = (float)v;
+                        } else {
+                            const int c = jp / sht::kChunk, kq = jp % sht::kChunk;
+                            const int grp = kq / 8, e = kq % 8, lane = grp * 16 + r;
+                            const long long idx = ana_off[m] + (((long long)(p * nt + lt) * JC + c) * 64 + lane) * 8 + e;
+                            split_store(v, s, ana_hi[idx], ana_lo[idx]);
+                        }
+                    }
+                }
+    }
+
+    // ------------------------------------------------------------- FFT tables
+    std::vector<int> fac((size_t)n * sht::kMaxFactors, 1), nfac(n);
+    std::vector<long long> tw_off(n);
+    long long tw_total = 0;
+    for (int j = 0; j < n; ++j) {
+        nfac[j] = factorize(g.nlon[j], &fac[(size_t)j * sht::kMaxFactors]);
+        tw_off[j] = tw_total;
+        tw_total += g.nlon[j];
+    }
+    std::vector<float> tw(2 * tw_total);
+    for (int j = 0; j < n; ++j) {
+        const int N = g.nlon[j];
+        for (int k = 0; k < N; ++k) {
+            const double a = -2.0 * M_PI * (double)k / (double)N;
+            tw[2 * (tw_off[j] + k)] = (float)std::cos(a);
+            tw[2 * (tw_off[j] + k) + 1] = (float)std::sin(a);
+        }
+    }
+    std::vector<double> ring_cost(n);
+    for (int j = 0; j < n; ++j) {
+        double c = 0;
+        for (int f = 0; f < nfac[j]; ++f) {
+            const int r = fac[(size_t)j * sht::kMaxFactors + f];
+            c += r <= 5 ? 1.0 : (double)r;
+        }
+        ring_cost[j] = c * g.nlon[j];
+    }
+    std::vector<int> ring_order(n);
+    std::iota(ring_order.begin(), ring_order.end(), 0);
+    std::stable_sort(ring_order.begin(), ring_order.end(), [&](int a, int b) { return ring_cost[a] > ring_cost[b]; });
+    int kg = (65536 / 8 - g.nlon_max) / (2 * g.nlon_max);
+    kg = std::max(1, std::min(8, kg));
+    P.kg = kg;
+    P.fft_smem = (g.nlon_max + 2 * kg * g.nlon_max) * 8;
+    if (P.fft_smem > 160 * 1024) return fail(SHT_ERR_UNSUPPORTED, "ring too long for the LDS FFT");
+
+    // --------------------------------------------------------------- tiles
+    std::vector<LegTile> syn_t, ana_t;
+    std::vector<double> syn_c, ana_c;
+    for (int m = 0; m < mmax; ++m) {
+        const int J = n - g.j0[m], L = lmax - m;
+        for (int b = 0; b * sht::kRingsPerWG < J; ++b) {
+            syn_t.push_back({m, b});
+            syn_c.push_back((double)std::min(sht::kRingsPerWG, J - b * sht::kRingsPerWG) * L);
+        }
+        for (int b = 0; b * sht::kDegPerWG < L; ++b) {
+            ana_t.push_back({m, b});
+            ana_c.push_back((double)std::min(sht::kDegPerWG, L - b * sht::kDegPerWG) * std::max(J, 1));
+        }
+    }
+    auto sort_tiles = [](std::vector<LegTile> &t, const std::vector<double> &c) {
+        std::vector<size_t> idx(t.size());
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return c[a] > c[b]; });
+        std::vector<LegTile> out(t.size());
+        for (size_t i = 0; i < idx.size(); ++i) out[i] = t[idx[i]];
+        t.swap(out);
+    };
+    sort_tiles(syn_t, syn_c);
+    sort_tiles(ana_t, ana_c);
+
+    // ---------------------------------------------------- UV_from_vordiv factors
+    std::vector<float> uvx(g.nharm), uv1(g.nharm), uv2(g.nharm);
+    std::vector<int> l_of(g.nharm);
+    auto eps = [](double l, double m) { return l > 0 ? std::sqrt((l * l - m * m) / (4.0 * l * l - 1.0)) : 0.0; };
+    for (int m = 0; m < mmax; ++m)
+        for (int l = m; l < lmax; ++l) {
+            const long long i = g.lm_off[m] + (l - m);
+            l_of[i] = l;
+            uvx[i] = l > 0 ? (float)(-(double)m / ((double)l * (l + 1))) : 0.f;
+            uv1[i] = l > 0 ? (float)(eps(l, m) / l) : 0.f;
+            uv2[i] = l < lmax - 1 ? (float)(eps(l + 1, m) / (l + 1)) : 0.f;
+        }
+
+    // --------------------------------------------------------------- upload
+    std::vector<float> coslat_f(coslat_inv.begin(), coslat_inv.end());
+    P.n = n; P.lmax = lmax; P.mmax = mmax; P.nharm = g.nharm; P.npoints = g.npoints;
+    if ((rc = dev_upload(A, g.nlon, &P.nlon))) return rc;
+    if ((rc = dev_upload(A, g.mt, &P.mt))) return rc;
+    if ((rc = dev_upload(A, g.fboff, &P.fboff))) return rc;
+    if ((rc = dev_upload(A, g.ring_n, &P.ring_n))) return rc;
+    if ((rc = dev_upload(A, g.ring_s, &P.ring_s))) return rc;
+    if ((rc = dev_upload(A, coslat_f, &P.coslat_inv))) return rc;
+    if ((rc = dev_upload(A, ring_order, &P.ring_order))) return rc;
+    if ((rc = dev_upload(A, g.j0, &P.j0))) return rc;
+    if ((rc = dev_upload(A, g.lm_off, &P.lm_off))) return rc;
+    if ((rc = dev_upload(A, syn_off, &P.syn_off))) return rc;
+    if ((rc = dev_upload(A, ana_off, &P.ana_off))) return rc;
+    if ((rc = dev_upload(A, syn_soff, &P.syn_soff))) return rc;
+    if ((rc = dev_upload(A, ana_soff, &P.ana_soff))) return rc;
+    if (split) {
+        const uint16_t *p16;
+        if ((rc = dev_upload(A, syn_hi, &p16))) return rc;
+        P.syn_hi = p16;
+        if ((rc = dev_upload(A, syn_lo, &p16))) return rc;
+        P.syn_lo = p16;
+        if ((rc = dev_upload(A, ana_hi, &p16))) return rc;
+        P.ana_hi = p16;
+        if ((rc = dev_upload(A, ana_lo, &p16))) return rc;
+        P.ana_lo = p16;
+        if ((rc = dev_upload(A, syn_sinv, &P.syn_sinv))) return rc;
+        if ((rc = dev_upload(A, ana_sinv, &P.ana_sinv))) return rc;
+    } else {
+        if ((rc = dev_upload(A, syn_f32, &P.syn_f32))) return rc;
+        if ((rc = dev_upload(A, ana_f32, &P.ana_f32))) return rc;
+    }
+    if ((rc = dev_upload(A, tw, &P.twiddle))) return rc;
+    if ((rc = dev_upload(A, tw_off, &P.tw_off))) return rc;
+    if ((rc = dev_upload(A, fac, &P.fac))) return rc;
+    if ((rc = dev_upload(A, nfac, &P.nfac))) return rc;
+    if ((rc = dev_upload(A, uvx, &P.uv_x))) return rc;
+    if ((rc = dev_upload(A, uv1, &P.uv_1))) return rc;
+    if ((rc = dev_upload(A, uv2, &P.uv_2))) return rc;
+    if ((rc = dev_upload(A, l_of, &P.l_of))) return rc;
+    if ((rc = dev_upload(A, syn_t, &pl->syn_tiles))) return rc;
+    if ((rc = dev_upload(A, ana_t, &pl->ana_tiles))) return rc;
+    pl->nsyn = (int)syn_t.size();
+    pl->nana = (int)ana_t.size();
+    const size_t fb_bytes = (size_t)g.fb_rows * d.nlayers * 16;
+    hipError_t e = hipMalloc(&pl->fbuf, fb_bytes);
+    if (e != hipSuccess) return fail(SHT_ERR_ALLOC, std::string("hipMalloc scratch: ") + hipGetErrorString(e));
+    A.push_back(pl->fbuf);
+
+    long long bytes = (long long)fb_bytes;
+    bytes += split ? 2 * 2 * (syn_total + ana_total) + 4 * (syn_stotal + ana_stotal) : 4 * (syn_total + ana_total);
+    bytes += 8 * tw_total + 16LL * g.nharm + 64LL * n + 48LL * mmax;
+    pl->device_bytes = bytes;
+    return SHT_OK;
+}
+
+}  // namespace
+
+// ================================================================== C ABI
+
+extern "C" {
+
+const char *sht_last_error(void) { return g_err.c_str(); }
+
+const char *sht_version(void) { return "libsht 0.1.0 gfx950"; }
+
+int sht_plan_sizes(const sht_desc *desc, sht_sizes *sizes) {
+    if (!sizes) return fail(SHT_ERR_ARG, "null sizes");
+    Geometry g;
+    int rc = make_geometry(desc, g);
+    if (rc) return rc;
+    sizes->npoints = g.npoints;
+    sizes->nharm = g.nharm;
+    sizes->nlat = g.nlat;
+    sizes->nlon_max = g.nlon_max;
+    sizes->nfreq_total = g.fb_rows;
+    // Λ (two directions, padded) + scratch, estimated without building
+    long long lam = 0;
+    for (int m = 0; m < g.mmax; ++m) {
+        const int J = g.n - g.j0[m], L = g.lmax - m, Le = (L + 1) / 2;
+        lam += 2LL * ((J + 15) / 16) * ((Le + 31) / 32) * 512;
+        lam += 2LL * 2 * ((L + 63) / 64) * ((J + 31) / 32) * 512;
+    }
+    sizes->device_bytes = lam * 4 + g.fb_rows * (long long)desc->nlayers * 16;
+    return SHT_OK;
+}
+
+int sht_plan_create(const sht_desc *desc, sht_plan **plan) {
+    if (!plan) return fail(SHT_ERR_ARG, "null plan pointer");
+    *plan = nullptr;
+    sht_plan *pl = new (std::nothrow) sht_plan();
+    if (!pl) return fail(SHT_ERR_ALLOC, "host allocation failed");
+    int rc = make_geometry(desc, pl->g);
+    if (rc) { delete pl; return rc; }
+    pl->d = *desc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        delete pl;
+        return fail(SHT_ERR_HIP, "no HIP device available");
+    }
+    if (desc->device < 0 || desc->device >= ndev) {
+        delete pl;
+        return fail(SHT_ERR_ARG, "device ordinal out of range");
+    }
+    DeviceGuard guard(desc->device);
+    rc = build_plan(pl);
+    if (rc) { free_plan(pl); return rc; }
+    *plan = pl;
+    return SHT_OK;
+}
+
+int sht_plan_destroy(sht_plan *plan) {
+    if (!plan) return SHT_OK;
+    DeviceGuard guard(plan->d.device);
+    (void)hipDeviceSynchronize();
+    free_plan(plan);
+    return SHT_OK;
+}
+
+static int check_call(sht_plan *plan, const void *a, const void *b, int K) {
+    if (!plan) return fail(SHT_ERR_ARG, "null plan");
+    if (!a || !b) return fail(SHT_ERR_ARG, "null data pointer");
+    if (K < 1 || K > plan->d.nlayers)
+        return fail(SHT_ERR_DIM, "DimensionMismatch: K=" + std::to_string(K) + " columns, plan holds nlayers=" +
+                                     std::to_string(plan->d.nlayers));
+    return SHT_OK;
+}
+
+int sht_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *field, int64_t ld_grid, int32_t K,
+                  int32_t unscale_coslat, void *stream) {
+    int rc = check_call(plan, coeffs, field, K);
+    if (rc) return rc;
+    if (ld_spec < plan->g.nharm) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_spec < nharm");
+    if (ld_grid < plan->g.npoints) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_grid < npoints");
+    DeviceGuard guard(plan->d.device);
+    {
+        ProfScope ps(plan, SHT_KERNEL_LEGENDRE_SYNTHESIS, stream);
+        rc = sht::launch_legendre_synthesis(plan->P, plan->d.precision, plan->syn_tiles, plan->nsyn, coeffs, ld_spec,
+                                            plan->fbuf, K, unscale_coslat ? 1 : 0, stream);
+    }
+    if (rc) return fail(SHT_ERR_HIP, std::string("legendre synthesis launch: ") + hipGetErrorString((hipError_t)rc));
+    {
+        ProfScope ps(plan, SHT_KERNEL_FFT_SYNTHESIS, stream);
+        rc = sht::launch_fft_synthesis(plan->P, plan->fbuf, field, ld_grid, K, stream);
+    }
+    if (rc) return fail(SHT_ERR_HIP, std::string("fft synthesis launch: ") + hipGetErrorString((hipError_t)rc));
+    return SHT_OK;
+}
+
+int sht_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void *coeffs, int64_t ld_spec, int32_t K,
+                 void *stream) {
+    int rc = check_call(plan, field, coeffs, K);
+    if (rc) return rc;
+    if (ld_spec < plan->g.nharm) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_spec < nharm");
+    if (ld_grid < plan->g.npoints) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_grid < npoints");
+    DeviceGuard guard(plan->d.device);
+    {
+        ProfScope ps(plan, SHT_KERNEL_FFT_ANALYSIS, stream);
+        rc = sht::launch_fft_analysis(plan->P, field, ld_grid, plan->fbuf, K, stream);
+    }
+    if (rc) return fail(SHT_ERR_HIP, std::string("fft analysis launch: ") + hipGetErrorString((hipError_t)rc));
+    {
+        ProfScope ps(plan, SHT_KERNEL_LEGENDRE_ANALYSIS, stream);
+        rc = sht::launch_legendre_analysis(plan->P, plan->d.precision, plan->ana_tiles, plan->nana, plan->fbuf,
+                                           coeffs, ld_spec, K, stream);
+    }
+    if (rc) return fail(SHT_ERR_HIP, std::string("legendre analysis launch: ") + hipGetErrorString((hipError_t)rc));
+    return SHT_OK;
+}
+
+int sht_uv_from_vordiv(sht_plan *plan, const void *vor, const void *div, void *U, void *V, int64_t ld, int32_t K,
+                       double radius, void *stream) {
+    int rc = check_call(plan, vor, div, K);
+    if (rc) return rc;
+    if (!U || !V) return fail(SHT_ERR_ARG, "null data pointer");
+    if (ld < plan->g.nharm) return fail(SHT_ERR_DIM, "DimensionMismatch: ld < nharm");
+    DeviceGuard guard(plan->d.device);
+    rc = sht::launch_uv_from_vordiv(plan->P, vor, div, U, V, ld, K, (float)radius, stream);
+    if (rc) return fail(SHT_ERR_HIP, std::string("UV_from_vordiv launch: ") + hipGetErrorString((hipError_t)rc));
+    return SHT_OK;
+}
+
+int sht_profile_begin(sht_plan *plan, int32_t max_calls) {
+    if (!plan || max_calls < 1) return fail(SHT_ERR_ARG, "bad arguments");
+    DeviceGuard guard(plan->d.device);
+    Profile &pr = plan->prof;
+    for (hipEvent_t e : pr.ev) (void)hipEventDestroy(e);
+    pr = Profile();
+    pr.ev.resize((size_t)SHT_NKERNELS * max_calls * 2);
+    for (auto &e : pr.ev) {
+        if (hipEventCreate(&e) != hipSuccess) return fail(SHT_ERR_HIP, "hipEventCreate failed");
+    }
+    pr.max_calls = max_calls;
+    pr.on = true;
+    return SHT_OK;
+}
+
+int sht_profile_end(sht_plan *plan, double *avg_ms, int32_t *counts) {
+    if (!plan || !avg_ms || !counts) return fail(SHT_ERR_ARG, "bad arguments");
+    DeviceGuard guard(plan->d.device);
+    Profile &pr = plan->prof;
+    int rc = SHT_OK;
+    for (int k = 0; k < SHT_NKERNELS; ++k) {
+        double tot = 0.0;
+        for (int c = 0; c < pr.ncalls[k]; ++c) {
+            hipEvent_t a = pr.ev[((size_t)k * pr.max_calls + c) * 2], b = pr.ev[((size_t)k * pr.max_calls + c) * 2 + 1];
+            float ms = 0.f;
+            if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+                rc = fail(SHT_ERR_HIP, "event timing failed");
+                break;
+            }
+            tot += ms;
+        }
+        counts[k] = pr.ncalls[k];
+        avg_ms[k] = pr.ncalls[k] ? tot / pr.ncalls[k] : 0.0;
+    }
+    for (hipEvent_t e : pr.ev) (void)hipEventDestroy(e);
+    pr = Profile();
+    return rc;
+}
+
+int sht_gauss_legendre(int32_t nlat, double *nodes, double *weights) {
+    if (nlat < 1 || !nodes || !weights) return fail(SHT_ERR_ARG, "bad arguments");
+    std::vector<double> x, w;
+    gauss_legendre(nlat, x, w);
+    std::memcpy(nodes, x.data(), sizeof(double) * nlat);
+    std::memcpy(weights, w.data(), sizeof(double) * nlat);
+    return SHT_OK;
+}
+
+int sht_legendre_column(int32_t lmax, int32_t mmax, double x, double *out) {
+    if (mmax < 1 || lmax < mmax || !out) return fail(SHT_ERR_ARG, "bad arguments");
+    const double s = std::sqrt(std::max(0.0, 1.0 - x * x));
+    double pmm = 0.0;
+    long long off = 0;
+    for (int m = 0; m < mmax; ++m) {
+        pmm = next_pmm(pmm, m, s);
+        legendre_order(lmax, m, x, pmm, out + off);
+        off += lmax - m;
+    }
+    return SHT_OK;
+}
+
+}  // extern "C"

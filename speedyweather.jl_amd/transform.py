@@ -1,0 +1,178 @@
+"""SpectralTransform and transform! on the MI355X engine.
+
+Mirrors SpeedyTransforms' API (SpeedyTransforms/src/spectral_transform.jl):
+  SpectralTransform(spectrum, grid; nlayers, LegendreShortcut)     :118-302
+  transform!(field, coeffs, S; unscale_coslat)  ->  transform_(field, coeffs, S, ...)  :515-570
+  transform!(coeffs, field, S)                  ->  transform_(coeffs, field, S)       :579-631
+  transform(field, S) / transform(coeffs, S)    ->  transform(...)                     :638-682
+  UV_from_vordiv!(U, V, vor, div, S; radius)    ->  UV_from_vordiv_(...)   spectral_gradients.jl:309-329
+Every call goes through libsht's C ABI with device pointers on the caller's current
+torch stream.  There is no CPU fallback: data must live on the plan's GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib
+from ._lib import DimensionMismatch, check, lib
+from .grids import Field, FullGaussianGrid, OctahedralGaussianGrid, get_nlat_half
+from .spectrum import LowerTriangularArray, Spectrum
+
+
+class SpectralTransform:
+    """Precomputed transform between `spectrum` and `grid` for up to `nlayers` columns.
+
+    precision: "fp32" (FP32 products and accumulation), "fp16x3" (scaled FP16 hi/lo split,
+    three MFMA products, FP32 accumulation) or "bf16x3" (BF16 hi/lo split)."""
+
+    def __init__(self, spectrum: Spectrum, grid, nlayers: int = 1, LegendreShortcut: str = "linear",
+                 precision: str = "fp32", device: int | None = None):
+        if grid.kind not in _lib.GRIDS:
+            raise NotImplementedError(f"{type(grid).__name__} is not supported by the MI355X engine")
+        self.spectrum, self.grid = spectrum, grid
+        self.nlayers = int(nlayers)
+        self.LegendreShortcut = LegendreShortcut
+        self.precision = precision
+        if device is None:
+            device = torch.cuda.current_device() if torch.cuda.is_available() else 0
+        self.device = int(device)
+        self._desc = _lib.ShtDesc(_lib.GRIDS[grid.kind], grid.nlat_half, spectrum.lmax, spectrum.mmax,
+                                  self.nlayers, _lib.SHORTCUTS[LegendreShortcut],
+                                  _lib.PRECISIONS[precision], self.device)
+        self._plan = C.c_void_p()
+        check(lib().sht_plan_create(C.byref(self._desc), C.byref(self._plan)), "sht_plan_create")
+
+    @classmethod
+    def from_grid(cls, grid, truncation: int = 0, dealiasing: float = 2.0, one_more_degree: bool = False,
+                  **kw) -> "SpectralTransform":
+        """SpectralTransform(grid; truncation, dealiasing, one_more_degree) (:305-320)."""
+        from .grids import get_truncation
+        truncation = truncation if truncation > 0 else get_truncation(grid.nlat_half, dealiasing)
+        return cls(Spectrum(truncation + int(one_more_degree), truncation), grid, **kw)
+
+    @classmethod
+    def from_spectrum(cls, spectrum: Spectrum, Grid=OctahedralGaussianGrid, nlat_half: int = 0,
+                      dealiasing: float = 2.0, **kw) -> "SpectralTransform":
+        """SpectralTransform(spectrum; Grid, nlat_half, dealiasing) (:323-334)."""
+        nlat_half = nlat_half if nlat_half > 0 else get_nlat_half(spectrum.mmax, dealiasing)
+        return cls(spectrum, Grid(nlat_half), **kw)
+
+    def close(self):
+        if getattr(self, "_plan", None) and self._plan.value:
+            lib().sht_plan_destroy(self._plan)
+            self._plan = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def norm_sphere(self) -> float:
+        return 2.0 * 3.141592653589793 ** 0.5
+
+    def __repr__(self):
+        return (f"SpectralTransform({self.spectrum}, {self.grid}, nlayers={self.nlayers}, "
+                f"precision={self.precision!r})")
+
+    # -------------------------------------------------------------- checks
+    def _stream(self):
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _check_tensor(self, t: torch.Tensor, n: int, dtype, what: str):
+        if not t.is_cuda or t.device.index != self.device:
+            raise ValueError(f"{what} must be on cuda:{self.device} (the MI355X engine has no CPU path)")
+        if t.dtype != dtype:
+            raise TypeError(f"{what} has dtype {t.dtype}, transform needs {dtype}")
+        t2 = t if t.dim() == 2 else t.unsqueeze(0)
+        if t2.stride(-1) != 1 or (t2.shape[0] > 1 and t2.stride(0) < n):
+            raise ValueError(f"{what} must be contiguous along its last dimension")
+        K = t2.shape[0]
+        ld = t2.stride(0) if K > 1 else n
+        return t2, K, ld
+
+    def ismatching(self, x) -> bool:
+        """Architectures.ismatching (spectral_transform.jl:387-403)."""
+        if isinstance(x, LowerTriangularArray):
+            return x.spectrum == self.spectrum and x.nlayers <= self.nlayers
+        if isinstance(x, Field):
+            return x.grid == self.grid and x.nlayers <= self.nlayers
+        return False
+
+
+def _mismatch(S: SpectralTransform, x) -> DimensionMismatch:
+    if isinstance(x, LowerTriangularArray):
+        return DimensionMismatch(f"SpectralTransform for {S.spectrum.lmax}x{S.spectrum.mmax}x{S.nlayers} "
+                                 f"LowerTriangularArrays with {x.spectrum}, K={x.nlayers} do not match.")
+    return DimensionMismatch(f"SpectralTransform for {S.grid} x {S.nlayers} with field on {x.grid}, "
+                             f"K={x.nlayers} do not match.")
+
+
+def transform_(out, inp, S: SpectralTransform, unscale_coslat: bool = False):
+    """In-place transform (transform!): Field <- LowerTriangularArray is spectral->grid
+    (optionally dividing by cos(lat) on the fly), LowerTriangularArray <- Field is
+    grid->spectral.  Returns `out`."""
+    if isinstance(out, Field) and isinstance(inp, LowerTriangularArray):
+        for x in (out, inp):
+            if not S.ismatching(x):
+                raise _mismatch(S, x)
+        if out.nlayers != inp.nlayers:
+            raise DimensionMismatch(f"field K={out.nlayers} and coeffs K={inp.nlayers} differ")
+        coeffs = inp.data
+        if coeffs.dtype == torch.complex128:  # spec->grid accepts Complex{Float64} coefficients
+            coeffs = coeffs.to(torch.complex64)
+        c2, K, ld_spec = S._check_tensor(coeffs, S.spectrum.nharm, torch.complex64, "coeffs")
+        f2, _, ld_grid = S._check_tensor(out.data, S.grid.npoints, torch.float32, "field")
+        check(lib().sht_synthesis(S._plan, c2.data_ptr(), ld_spec, f2.data_ptr(), ld_grid, K,
+                                  int(bool(unscale_coslat)), S._stream()), "sht_synthesis")
+        return out
+    if isinstance(out, LowerTriangularArray) and isinstance(inp, Field):
+        if unscale_coslat:
+            raise TypeError("unscale_coslat applies to spectral->grid transforms only")
+        for x in (out, inp):
+            if not S.ismatching(x):
+                raise _mismatch(S, x)
+        if out.nlayers != inp.nlayers:
+            raise DimensionMismatch(f"coeffs K={out.nlayers} and field K={inp.nlayers} differ")
+        f2, K, ld_grid = S._check_tensor(inp.data, S.grid.npoints, torch.float32, "field")
+        c2, _, ld_spec = S._check_tensor(out.data, S.spectrum.nharm, torch.complex64, "coeffs")
+        check(lib().sht_analysis(S._plan, f2.data_ptr(), ld_grid, c2.data_ptr(), ld_spec, K,
+                                 S._stream()), "sht_analysis")
+        return out
+    raise TypeError("transform_ needs (Field, LowerTriangularArray) or (LowerTriangularArray, Field)")
+
+
+def transform(x, S: SpectralTransform, unscale_coslat: bool = False):
+    """Allocating transform (spectral_transform.jl:638-682)."""
+    if isinstance(x, LowerTriangularArray):
+        shape = (S.grid.npoints,) if x.data.dim() == 1 else (x.nlayers, S.grid.npoints)
+        out = Field(torch.empty(shape, dtype=torch.float32, device=x.data.device), S.grid)
+        return transform_(out, x, S, unscale_coslat=unscale_coslat)
+    if isinstance(x, Field):
+        shape = (S.spectrum.nharm,) if x.data.dim() == 1 else (x.nlayers, S.spectrum.nharm)
+        out = LowerTriangularArray(torch.empty(shape, dtype=torch.complex64, device=x.data.device), S.spectrum)
+        return transform_(out, x, S)
+    raise TypeError("transform needs a Field or a LowerTriangularArray")
+
+
+def UV_from_vordiv_(U: LowerTriangularArray, V: LowerTriangularArray, vor: LowerTriangularArray,
+                    div: LowerTriangularArray, S: SpectralTransform, radius: float = 1.0):
+    """U, V = (u, v) cos(lat) from vorticity and divergence (spectral_gradients.jl:309-366)."""
+    for x in (U, V, vor, div):
+        if not S.ismatching(x):
+            raise _mismatch(S, x)
+    K = vor.nlayers
+    if not (U.nlayers == V.nlayers == div.nlayers == K):
+        raise DimensionMismatch("U, V, vor, div must have the same number of layers")
+    ts = [S._check_tensor(x.data, S.spectrum.nharm, torch.complex64, nm)
+          for x, nm in ((vor, "vor"), (div, "div"), (U, "U"), (V, "V"))]
+    lds = {t[2] for t in ts}
+    if len(lds) != 1:
+        raise ValueError("U, V, vor, div must share one column stride")
+    check(lib().sht_uv_from_vordiv(S._plan, ts[0][0].data_ptr(), ts[1][0].data_ptr(), ts[2][0].data_ptr(),
+                                   ts[3][0].data_ptr(), ts[0][2], K, float(radius), S._stream()),
+          "sht_uv_from_vordiv")
+    return U, V
