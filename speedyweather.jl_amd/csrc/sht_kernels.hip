@@ -476,6 +476,102 @@ __device__ void fft_pass(const float2 *in, float2 *out, const float2 *tw, int N,
     }
 }
 
+// Radix-r pass (r odd, >= 11) as two small GEMMs on MFMA.  With the twiddled inputs
+// x_q of one butterfly, a_t = x_t + x_{r-t}, b_t = x_t - x_{r-t} (t = 1..h, h = (r-1)/2):
+//   T1[s] = sum_t cos(2 pi s t / r) a_t   (s = 0..h),   T2[s] = sum_t sin(2 pi s t / r) b_t
+//   y_0 = x_0 + T1[0],  y_s = x_0 + T1[s] -+ i T2[s],  y_{r-s} = x_0 + T1[s] +- i T2[s]
+// (upper signs forward).  GEMM rows = s, columns = (butterfly, re/im), K = t; the cos/sin
+// operand comes from the ring's twiddle table in LDS at index (s t mod r) N/r.
+template <bool INV>
+__device__ void fft_pass_mfma(float2 *in, float2 *out, const float2 *tw, int N, int r, int l, int ncol,
+                              int tid) {
+    const int M = N / (l * r);
+    const int h = (r - 1) >> 1;
+    const int nbf = N / r;          // butterflies per column
+    const int nseq = ncol * nbf;
+    // twiddles (only for u > 0) and the pair sums / differences, in place
+    for (int it = tid; it < nseq * h; it += kFFTThreads) {
+        const int sq = it / h, t = it - sq * h + 1;
+        const int col = sq / nbf, bt = sq - col * nbf;
+        const int b = bt % M, u = bt / M;
+        float2 *x = in + col * N + b + M * r * u;
+        float2 xt = x[M * t], xr = x[M * (r - t)];
+        if (u > 0) {
+            xt = cmul(xt, twid<INV>(tw, t * u * M));
+            xr = cmul(xr, twid<INV>(tw, (r - t) * u * M));
+        }
+        x[M * t] = cadd(xt, xr);
+        x[M * (r - t)] = csub(xt, xr);
+    }
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6, row = lane & 15, grp = lane >> 4;
+    const int Mt = (h + 16) >> 4;             // rows 0..h
+    const int Nt = (2 * nseq + 15) >> 4;      // real columns
+    const int Kt = (h + 3) >> 2;
+    const int step = N / r;
+    for (int tile = wave; tile < Mt * Nt; tile += kFFTThreads / 64) {
+        const int mt = tile % Mt, nt = tile / Mt;
+        const int sA = mt * 16 + row;                 // A-operand row
+        const int c = nt * 16 + row;                  // B-operand / output column
+        const int sq = c >> 1, reim = c & 1;
+        const bool cval = sq < nseq;
+        int base = 0;
+        if (cval) {
+            const int col = sq / nbf, bt = sq - col * nbf;
+            base = col * N + bt % M + M * r * (bt / M);
+        }
+        const float *xf = reinterpret_cast<const float *>(in + base) + reim;
+        f4 t1 = {0.f, 0.f, 0.f, 0.f}, t2 = {0.f, 0.f, 0.f, 0.f};
+        int idx = (sA * (grp + 1)) % r;
+        const int inc = (4 * sA) % r;
+        for (int ks = 0; ks < Kt; ++ks) {
+            const int t = ks * 4 + grp + 1;
+            float cs = 0.f, sn = 0.f, av = 0.f, bv = 0.f;
+            if (t <= h) {
+                if (sA <= h) {
+                    const float2 w = tw[idx * step];
+                    cs = w.x;
+                    sn = -w.y;
+                }
+                if (cval) {
+                    av = xf[2 * M * t];
+                    bv = xf[2 * M * (r - t)];
+                }
+            }
+            t1 = mfma_f32(cs, av, t1);
+            t2 = mfma_f32(sn, bv, t2);
+            idx += inc;
+            if (idx >= r) idx -= r;
+        }
+        // lane holds rows s = 16 mt + 4 grp + i, column c; pair re/im through lane^1
+        float2 x0 = make_float2(0.f, 0.f);
+        int ob = 0;
+        if (cval) {
+            x0 = in[base];
+            const int col = sq / nbf, bt = sq - col * nbf;
+            ob = col * N + bt % M + M * (bt / M);  // out[b + M(u + l s)] = ob + M l s
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float a1 = t1[i], a2 = t2[i];
+            const float p1 = __shfl_xor(a1, 1), p2 = __shfl_xor(a2, 1);
+            const int s = mt * 16 + 4 * grp + i;
+            if (!cval || s > h) continue;
+            // re lane (reim 0): T1re = a1, T1im = p1, T2re = a2, T2im = p2; im lane swapped
+            const float T1re = reim ? p1 : a1, T1im = reim ? a1 : p1;
+            const float T2re = reim ? p2 : a2, T2im = reim ? a2 : p2;
+            const float sg = INV ? -1.f : 1.f;
+            if (s == 0) {
+                if (reim == 0) out[ob] = make_float2(x0.x + T1re, x0.y + T1im);
+            } else if (reim == 0) {
+                out[ob + M * l * s] = make_float2(x0.x + T1re + sg * T2im, x0.y + T1im - sg * T2re);
+            } else {
+                out[ob + M * l * (r - s)] = make_float2(x0.x + T1re - sg * T2im, x0.y + T1im + sg * T2re);
+            }
+        }
+    }
+}
+
 // Runs all passes; returns the buffer holding the result.
 template <bool INV>
 __device__ float2 *fft_run(float2 *b0, float2 *b1, const float2 *tw, const int *fac, int nf, int N,
@@ -484,7 +580,10 @@ __device__ float2 *fft_run(float2 *b0, float2 *b1, const float2 *tw, const int *
     float2 *in = b0, *out = b1;
     for (int f = 0; f < nf; ++f) {
         const int r = fac[f];
-        fft_pass<INV>(in, out, tw, N, r, l, ncol, tid);
+        if (r >= 11)
+            fft_pass_mfma<INV>(in, out, tw, N, r, l, ncol, tid);
+        else
+            fft_pass<INV>(in, out, tw, N, r, l, ncol, tid);
         __syncthreads();
         l *= r;
         float2 *t = in;

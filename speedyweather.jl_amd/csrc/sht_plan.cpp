@@ -271,14 +271,26 @@ void free_plan(sht_plan *p) {
     delete p;
 }
 
+// Radices of the ring FFT, in pass order: odd primes >= 11 first (largest first; the
+// first pass needs no twiddles and these run as MFMA GEMMs), then 4s, 2, 3, 5, 7.
 int factorize(int N, int *fac) {
     int nf = 0, r = N;
+    std::vector<int> big;
+    int tmp = r;
+    for (int q : {2, 3, 5, 7})
+        while (tmp % q == 0) tmp /= q;
+    for (int p = 11; p <= tmp; p += 2) {  // tmp has no factor below 11: p divides only if prime
+        while (tmp % p == 0) {
+            big.push_back(p);
+            tmp /= p;
+        }
+    }
+    std::sort(big.begin(), big.end(), [](int a, int b) { return a > b; });
+    for (int p : big) { fac[nf++] = p; r /= p; }
     while (r % 4 == 0) { fac[nf++] = 4; r /= 4; }
-    const int small[3] = {2, 3, 5};
+    const int small[4] = {2, 3, 5, 7};
     for (int s : small)
         while (r % s == 0) { fac[nf++] = s; r /= s; }
-    for (int p = 7; r > 1; p += 2)
-        while (r % p == 0) { fac[nf++] = p; r /= p; }
     return nf;
 }
 
