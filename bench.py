@@ -86,6 +86,80 @@ def run_mode(sw, torch, precision, spectrum, grid, coeffs_host, steps, warmup):
             "kernel_ms": {KERNELS[i]: avg[i] for i in range(4)}, "roundtrip": out}
 
 
+def time_loop(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def bench_c2(sw, torch, steps, warmup):
+    """T127 (lmax=129, mmax=128) octahedral nlat_half=96, 8 layers x 4 fields: scalar round trip
+    (K=32) plus the vector path UV_from_vordiv (vor, div: K=8) -> U, V -> grid u, v with
+    unscale_coslat (K=16), as the model's transform step does it."""
+    sp, grid = sw.Spectrum(129, 128), sw.OctahedralGaussianGrid(96)
+    S = sw.SpectralTransform(sp, grid, nlayers=32, precision="fp32")
+    c = sw.LowerTriangularArray(torch.from_numpy(sw.seeded_spectrum(sp, 32, 5)).to(torch.complex64).cuda(), sp)
+    f = sw.Field(torch.empty((32, grid.npoints), device="cuda"), grid)
+    b = sw.LowerTriangularArray(torch.empty_like(c.data), sp)
+    uv = sw.LowerTriangularArray(torch.empty((16, sp.nharm), dtype=torch.complex64, device="cuda"), sp)
+    U = sw.LowerTriangularArray(uv.data[:8], sp)
+    V = sw.LowerTriangularArray(uv.data[8:], sp)
+    vor = sw.LowerTriangularArray(c.data[:8], sp)
+    div = sw.LowerTriangularArray(c.data[8:16], sp)
+    fuv = sw.Field(torch.empty((16, grid.npoints), device="cuda"), grid)
+
+    def step():
+        sw.transform_(f, c, S)
+        sw.transform_(b, f, S)
+        sw.UV_from_vordiv_(U, V, vor, div, S)
+        sw.transform_(fuv, uv, S, unscale_coslat=True)
+    dt = time_loop(torch, step, steps, warmup)
+    S.close()
+    return {"ms": dt * 1e3, "per_s": 1.0 / dt,
+            "workload": "T127 octahedral nlat_half=96: K=32 round trip + UV_from_vordiv (K=8) + U,V synthesis "
+                        "with unscale_coslat (K=16), FP32"}
+
+
+def bench_c4(sw, torch, steps, warmup):
+    """64 independent round trips, truncations from {T31,T63,T127,T255} (octahedral), layers
+    U{1..16} x fields U{1..4}, seed 1234, FP32: one batched launch per stage from a hipGraph,
+    against the same 64 round trips issued one transform call at a time."""
+    sizes = {31: (24, 33, 32), 63: (48, 65, 64), 127: (96, 129, 128), 255: (192, 257, 256)}
+    plans = {T: sw.SpectralTransform(sw.Spectrum(l, m), sw.OctahedralGaussianGrid(n), nlayers=64, precision="fp32")
+             for T, (n, l, m) in sizes.items()}
+    rng = np.random.default_rng(1234)
+    items = []
+    for i in range(64):
+        T = int(rng.choice([31, 63, 127, 255]))
+        K = int(rng.integers(1, 17)) * int(rng.integers(1, 5))
+        S = plans[T]
+        c = sw.LowerTriangularArray(torch.from_numpy(sw.seeded_spectrum(S.spectrum, K, 1000 + i))
+                                    .to(torch.complex64).cuda(), S.spectrum)
+        mid = sw.Field(torch.empty((K, S.grid.npoints), device="cuda"), S.grid)
+        out = sw.LowerTriangularArray(torch.empty_like(c.data), S.spectrum)
+        items.append((S, c, out, mid))
+    B = sw.SpectralBatch(items)
+    dt_batch = time_loop(torch, B.launch, steps, warmup)
+
+    def sequential():
+        for S, c, out, mid in items:
+            sw.transform_(mid, c, S)
+            sw.transform_(out, mid, S)
+    dt_seq = time_loop(torch, sequential, max(steps // 4, 5), max(warmup // 4, 2))
+    res = {"batch_ms": dt_batch * 1e3, "transforms_per_s": 64.0 / dt_batch, "sequential_ms": dt_seq * 1e3,
+           "speedup_vs_sequential": dt_seq / dt_batch, "uses_graph": B.uses_graph,
+           "total_columns": int(sum(it[1].nlayers for it in items))}
+    B.close()
+    for S in plans.values():
+        S.close()
+    return res
+
+
 def cpu_baseline(spectrum_o, grid_o, coeffs, seconds=10.0):
     """The oracle's FP32 C restatement of the reference CPU path on the host cores, timed on
     whole round trips of the same workload until `seconds` have elapsed (at least one)."""
@@ -205,6 +279,8 @@ def main():
         "max_rel_err_vs_fp64": {p: e["worst"] for p, e in errs.items()},
         "rel_err_per_field_band": {p: e["per_field_band"] for p, e in errs.items()},
     }
+    out["c2_t127_vector_path"] = bench_c2(sw, torch, args.steps, args.warmup)
+    out["c4_uneven_batch64"] = bench_c4(sw, torch, args.steps, args.warmup)
     if not args.no_cpu:
         cb = cpu_baseline(OSpectrum(lmax, mmax), OGrid("octahedral", n), coeffs, args.cpu_seconds)
         out["cpu_baseline"] = cb

@@ -108,6 +108,60 @@ int sht_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void *coeff
 int sht_uv_from_vordiv(sht_plan *plan, const void *vor, const void *div, void *U, void *V,
                        int64_t ld, int32_t K, double radius, void *stream);
 
+/* Uneven batches: many independent transforms (any mix of plans, i.e. truncations and
+ * grids, layer counts and fields) run as ONE launch per stage, their work tiles merged
+ * and ordered heaviest first, replayed from an explicitly built hipGraph.  No reference
+ * counterpart (the reference transforms one SpectralTransform per call and, on CUDA only,
+ * captures the Fourier stage in a graph: SpeedyTransforms/ext/gpu_graphs_common.jl:293-331).
+ * Items keep the sht_synthesis / sht_analysis conventions; a round trip writes its grid to
+ * `mid` and its coefficients to `out`.  All plans must share one precision mode and device;
+ * K <= the plan's nlayers.  The batch owns its Fourier scratch, so items on one plan do not
+ * race.  The pointers are fixed at creation: a launch replays the same transforms. */
+#define SHT_DIR_SYNTHESIS 0
+#define SHT_DIR_ANALYSIS 1
+#define SHT_DIR_ROUNDTRIP 2
+typedef struct sht_batch_item {
+    sht_plan *plan;
+    int32_t direction;      /* SHT_DIR_* */
+    int32_t K;
+    int32_t unscale_coslat; /* synthesis and round trip */
+    int32_t reserved;
+    const void *in;         /* synthesis, round trip: coefficients; analysis: grid */
+    int64_t ld_in;
+    void *mid;              /* round trip: grid in between; otherwise unused */
+    int64_t ld_mid;
+    void *out;              /* synthesis: grid; analysis, round trip: coefficients */
+    int64_t ld_out;
+} sht_batch_item;
+typedef struct sht_batch sht_batch;
+#define SHT_BATCH_NO_GRAPH 1             /* flag: launch the stages directly */
+#define SHT_BATCH_FORCE_GRAPH_FAILURE 2  /* flag for tests: treat graph instantiation as failed */
+int sht_batch_create(const sht_batch_item *items, int32_t n, int32_t flags, sht_batch **batch);
+int sht_batch_launch(sht_batch *batch, void *stream);
+/* 1 if launches replay a hipGraph, 0 if they fall back to direct kernel launches */
+int sht_batch_uses_graph(const sht_batch *batch);
+int sht_batch_destroy(sht_batch *batch);
+
+/* Learned column correction (north-star addition; no reference counterpart): an MLP
+ * 32 -> 128 -> 128 -> 32 with GELU per grid column (8 levels x {u, v, T, ln ps}), BF16
+ * weights with FP32 master copy and FP32 accumulation, trained with Adam on the mean
+ * squared error.  x, t, y, grad and loss are device pointers; x, t, y are B x 32 row-major.
+ * Parameters (sht_mlp_nparams() floats): W1[128][32] b1[128] W2[128][128] b2[128]
+ * W3[32][128] b3[32]. */
+typedef struct sht_mlp sht_mlp;
+int sht_mlp_nparams(void);
+int sht_mlp_create(int32_t device, int32_t max_batch, sht_mlp **mlp);
+int sht_mlp_set_params(sht_mlp *mlp, const float *host_params);
+int sht_mlp_get_params(sht_mlp *mlp, float *host_params);
+int sht_mlp_forward(sht_mlp *mlp, const float *x, float *y, int32_t B, void *stream);
+/* gradient of the loss w.r.t. the parameters (no update) and the loss */
+int sht_mlp_gradient(sht_mlp *mlp, const float *x, const float *t, int32_t B, float *grad, float *loss,
+                     void *stream);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) with learning rate lr; loss may be null */
+int sht_mlp_train_step(sht_mlp *mlp, const float *x, const float *t, int32_t B, float lr, float *loss,
+                       void *stream);
+int sht_mlp_destroy(sht_mlp *mlp);
+
 /* Per-kernel timing (no reference counterpart; the reference profiled ad hoc with
  * CUDA.@profile / BenchmarkTools, docs/dev/2026-08/legendre-gpu-optimization.md).
  * Between begin and end, the first max_calls launches of every kernel are bracketed by

@@ -5,10 +5,11 @@ from .grids import (Field, FullGaussianGrid, OctahedralGaussianGrid, get_nlat_ha
                     roundup_fft)
 from .spectrum import LowerTriangularArray, Spectrum
 from .synthetic import seeded_grid, seeded_spectrum
-from .transform import SpectralTransform, UV_from_vordiv_, transform, transform_
+from .transform import SpectralBatch, SpectralTransform, UV_from_vordiv_, transform, transform_
 
 __all__ = [
     "DimensionMismatch", "SHTError", "lib", "Field", "FullGaussianGrid", "OctahedralGaussianGrid",
     "get_nlat_half", "get_truncation", "roundup_fft", "LowerTriangularArray", "Spectrum",
-    "seeded_grid", "seeded_spectrum", "SpectralTransform", "UV_from_vordiv_", "transform", "transform_",
+    "seeded_grid", "seeded_spectrum", "SpectralBatch", "SpectralTransform", "UV_from_vordiv_", "transform",
+    "transform_",
 ]

@@ -23,7 +23,9 @@ PRECISIONS = {"fp32": 0, "fp16x3": 1, "bf16x3": 2}
 EXPORTS = (
     "sht_plan_create", "sht_plan_destroy", "sht_plan_sizes", "sht_synthesis", "sht_analysis",
     "sht_uv_from_vordiv", "sht_gauss_legendre", "sht_legendre_column", "sht_last_error",
-    "sht_version", "sht_profile_begin", "sht_profile_end",
+    "sht_version", "sht_profile_begin", "sht_profile_end", "sht_batch_create", "sht_batch_launch",
+    "sht_batch_uses_graph", "sht_batch_destroy", "sht_mlp_nparams", "sht_mlp_create", "sht_mlp_set_params",
+    "sht_mlp_get_params", "sht_mlp_forward", "sht_mlp_gradient", "sht_mlp_train_step", "sht_mlp_destroy",
 )
 
 
@@ -40,6 +42,17 @@ class ShtDesc(C.Structure):
     _fields_ = [("grid", C.c_int32), ("nlat_half", C.c_int32), ("lmax", C.c_int32),
                 ("mmax", C.c_int32), ("nlayers", C.c_int32), ("shortcut", C.c_int32),
                 ("precision", C.c_int32), ("device", C.c_int32)]
+
+
+class ShtBatchItem(C.Structure):
+    _fields_ = [("plan", C.c_void_p), ("direction", C.c_int32), ("K", C.c_int32),
+                ("unscale_coslat", C.c_int32), ("reserved", C.c_int32),
+                ("in_", C.c_void_p), ("ld_in", C.c_int64), ("mid", C.c_void_p), ("ld_mid", C.c_int64),
+                ("out", C.c_void_p), ("ld_out", C.c_int64)]
+
+
+SHT_DIR_SYNTHESIS, SHT_DIR_ANALYSIS, SHT_DIR_ROUNDTRIP = 0, 1, 2
+SHT_BATCH_NO_GRAPH, SHT_BATCH_FORCE_GRAPH_FAILURE = 1, 2
 
 
 class ShtSizes(C.Structure):
@@ -66,6 +79,17 @@ def lib() -> C.CDLL:
         L.sht_uv_from_vordiv.argtypes = [vp, vp, vp, vp, vp, i64, i32, C.c_double, vp]
         L.sht_gauss_legendre.argtypes = [i32, vp, vp]
         L.sht_legendre_column.argtypes = [i32, i32, C.c_double, vp]
+        L.sht_batch_create.argtypes = [C.POINTER(ShtBatchItem), i32, i32, C.POINTER(vp)]
+        L.sht_batch_launch.argtypes = [vp, vp]
+        L.sht_batch_uses_graph.argtypes = [vp]
+        L.sht_batch_destroy.argtypes = [vp]
+        L.sht_mlp_create.argtypes = [i32, i32, C.POINTER(vp)]
+        L.sht_mlp_set_params.argtypes = [vp, vp]
+        L.sht_mlp_get_params.argtypes = [vp, vp]
+        L.sht_mlp_forward.argtypes = [vp, vp, vp, i32, vp]
+        L.sht_mlp_gradient.argtypes = [vp, vp, vp, i32, vp, vp, vp]
+        L.sht_mlp_train_step.argtypes = [vp, vp, vp, i32, C.c_float, vp, vp]
+        L.sht_mlp_destroy.argtypes = [vp]
         L.sht_profile_begin.argtypes = [vp, i32]
         L.sht_profile_end.argtypes = [vp, vp, vp]
         L.sht_last_error.restype = C.c_char_p

@@ -1,0 +1,75 @@
+// Device-side helpers shared by the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include "sht_internal.h"
+
+namespace sht {
+
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16;
+typedef u16 u16x8 __attribute__((ext_vector_type(8)));
+
+// ------------------------------------------------------------------ MFMA helpers
+
+__device__ inline f4 mfma_f32(float a, float b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <int MODE>
+struct Split;
+
+template <>
+struct Split<kFP16x3> {
+    typedef _Float16 T;
+    typedef h8 V;
+    __device__ static inline f4 mma(V a, V b, f4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct Split<kBF16x3> {
+    typedef __bf16 T;
+    typedef b8 V;
+    __device__ static inline f4 mma(V a, V b, f4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+// Power-of-two scale that maps max|x| into [2^14, 2^15) (FP16 hi/lo), with its inverse.
+__device__ inline void fp16_scale(float amax, float &s, float &sinv) {
+    if (!(amax > 0.f) || !isfinite(amax)) {
+        s = 1.f;
+        sinv = 1.f;
+        return;
+    }
+    int e;
+    frexpf(amax, &e);  // amax = f 2^e, f in [0.5, 1)
+    int sh = 15 - e;
+    sh = sh > 120 ? 120 : (sh < -120 ? -120 : sh);
+    s = ldexpf(1.f, sh);
+    sinv = ldexpf(1.f, -sh);
+}
+
+
+__device__ inline float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// Exact x / d for 0 <= x < 2^16 and 1 <= d < 2^16 by a multiply-high with ceil(2^32 / d).
+struct FastDiv {
+    unsigned mul;
+    int d;
+    __device__ explicit FastDiv(int dd) : mul(dd > 1 ? 0xFFFFFFFFu / (unsigned)dd + 1u : 0u), d(dd) {}
+    __device__ int div(int x) const { return d == 1 ? x : (int)__umulhi((unsigned)x, mul); }
+};
+
+static inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace sht

@@ -1,0 +1,573 @@
+// gfx950 ring FFT of the spherical harmonic transform (Fourier stage).
+//
+// One complex FFT of length nlon per ring and column carries the northern ring in the
+// real part and the mirrored southern ring in the imaginary part: the per-ring
+// rfft / brfft pairs of SpeedyTransforms/src/fourier.jl:27-77,129-250 for both hemispheres
+// at once.  Mixed-radix Stockham autosort in LDS, decimation in time.  With
+// N = r_1 r_2 ... r_s, pass p of radix r (l = r_1...r_{p-1}, M = N/(l r)) maps, for
+// b in [0,M), u in [0,l):
+//   in[b + M(q + r u)], q < r   ->   out[b + M(u + l s)], s < r
+//   out = sum_q (w_N^{q u M} in_q) w_r^{q s},   w_N = exp(-/+ 2 pi i / N).
+// Every pass runs in place in one LDS buffer.  A first-pass prime >= 17 (the octahedral
+// rings nlon = 16 + 4j have prime factors up to 193 at T255) runs across the workgroup as
+// two small GEMMs on MFMA; all other passes (radices 2..19, register butterflies) run on
+// one column per wave with wave-local synchronisation only.
+#include "sht_device.h"
+
+namespace sht {
+
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ inline float2 mul_mi(float2 a) {
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <bool INV>
+__device__ inline float2 twid(const float2 *tw, int idx) {
+    const float2 t = tw[idx];
+    return INV ? make_float2(t.x, -t.y) : t;
+}
+
+__device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ inline float2 cfma(float2 a, float s, float2 c) { return make_float2(fmaf(a.x, s, c.x), fmaf(a.y, s, c.y)); }
+
+// DFT of R values in place (R = 2, 3, 4, 5, 7)
+template <int R, bool INV>
+__device__ inline void bfly(float2 *a) {
+    if constexpr (R == 2) {
+        const float2 t = a[0];
+        a[0] = cadd(t, a[1]);
+        a[1] = csub(t, a[1]);
+    } else if constexpr (R == 4) {
+        const float2 s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
+        const float2 s13 = cadd(a[1], a[3]), d13 = mul_mi<INV>(csub(a[1], a[3]));
+        a[0] = cadd(s02, s13);
+        a[1] = cadd(d02, d13);
+        a[2] = csub(s02, s13);
+        a[3] = csub(d02, d13);
+    } else if constexpr (R == 3) {
+        const float s3 = 0.86602540378443864676f;
+        const float2 s = cadd(a[1], a[2]);
+        const float2 d = mul_mi<INV>(csub(a[1], a[2]));
+        const float2 base = cfma(s, -0.5f, a[0]);
+        a[0] = cadd(a[0], s);
+        a[1] = cfma(d, s3, base);
+        a[2] = cfma(d, -s3, base);
+    } else if constexpr (R == 5) {
+        const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+        const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+        const float2 s14 = cadd(a[1], a[4]), d14 = mul_mi<INV>(csub(a[1], a[4]));
+        const float2 s23 = cadd(a[2], a[3]), d23 = mul_mi<INV>(csub(a[2], a[3]));
+        const float2 a0 = a[0];
+        const float2 b1 = cfma(s23, c2, cfma(s14, c1, a0));
+        const float2 b2 = cfma(s23, c1, cfma(s14, c2, a0));
+        const float2 e1 = cfma(d23, s2, cscale(d14, s1));
+        const float2 e2 = cfma(d23, -s1, cscale(d14, s2));
+        a[0] = cadd(a0, cadd(s14, s23));
+        a[1] = cadd(b1, e1);
+        a[4] = csub(b1, e1);
+        a[2] = cadd(b2, e2);
+        a[3] = csub(b2, e2);
+    } else {  // R == 7
+        const float c1 = 0.62348980185873353053f, c2 = -0.22252093395631440429f, c3 = -0.90096886790241912624f;
+        const float s1 = 0.78183148246802980871f, s2 = 0.97492791218182360702f, s3 = 0.43388373911755812048f;
+        const float2 a0 = a[0];
+        const float2 p1 = cadd(a[1], a[6]), q1 = mul_mi<INV>(csub(a[1], a[6]));
+        const float2 p2 = cadd(a[2], a[5]), q2 = mul_mi<INV>(csub(a[2], a[5]));
+        const float2 p3 = cadd(a[3], a[4]), q3 = mul_mi<INV>(csub(a[3], a[4]));
+        const float2 b1 = cfma(p3, c3, cfma(p2, c2, cfma(p1, c1, a0)));
+        const float2 b2 = cfma(p3, c1, cfma(p2, c3, cfma(p1, c2, a0)));
+        const float2 b3 = cfma(p3, c2, cfma(p2, c1, cfma(p1, c3, a0)));
+        const float2 e1 = cfma(q3, s3, cfma(q2, s2, cscale(q1, s1)));
+        const float2 e2 = cfma(q3, -s1, cfma(q2, -s3, cscale(q1, s2)));
+        const float2 e3 = cfma(q3, s2, cfma(q2, -s1, cscale(q1, s3)));
+        a[0] = cadd(a0, cadd(p1, cadd(p2, p3)));
+        a[1] = cadd(b1, e1);
+        a[6] = csub(b1, e1);
+        a[2] = cadd(b2, e2);
+        a[5] = csub(b2, e2);
+        a[3] = cadd(b3, e3);
+        a[4] = csub(b3, e3);
+    }
+}
+
+// DFT of an odd prime number R of values in place (R = 11, 13, 17, 19), by the symmetric
+// pairs a_t +- a_{R-t}; cos / sin of 2 pi k / R come from the ring's table tw at k N/R.
+template <int R, bool INV>
+__device__ inline void bfly_odd(float2 *a, const float2 *tw, int stepR) {
+    constexpr int H = (R - 1) / 2;
+    float cs[H + 1], sn[H + 1];
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        const float2 w = tw[k * stepR];
+        cs[k] = w.x;
+        sn[k] = -w.y;
+    }
+    float2 p[H], q[H];
+    float2 y0 = a[0];
+#pragma unroll
+    for (int t = 1; t <= H; ++t) {
+        p[t - 1] = cadd(a[t], a[R - t]);
+        q[t - 1] = mul_mi<INV>(csub(a[t], a[R - t]));
+        y0 = cadd(y0, p[t - 1]);
+    }
+#pragma unroll
+    for (int s = 1; s <= H; ++s) {
+        float2 b = a[0], e = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int t = 1; t <= H; ++t) {
+            const int k = (s * t) % R;
+            const float c = k <= H ? cs[k] : cs[R - k];
+            const float sg = k <= H ? sn[k] : -sn[R - k];
+            b = cfma(p[t - 1], c, b);
+            e = cfma(q[t - 1], sg, e);
+        }
+        a[s] = cadd(b, e);
+        a[R - s] = csub(b, e);
+    }
+    a[0] = y0;
+}
+
+// multiply by w8^k = exp(-+ i pi k / 4), k = 1, 2, 3
+template <int K, bool INV>
+__device__ inline float2 mul_w8(float2 a) {
+    constexpr float h = 0.70710678118654752440f;
+    if constexpr (K == 2) return mul_mi<INV>(a);
+    // forward: w8 = (1 - i) h, w8^3 = (-1 - i) h; inverse: conjugates
+    const float2 m = mul_mi<INV>(a);  // a * w8^2
+    if constexpr (K == 1) return make_float2(h * (a.x + m.x), h * (a.y + m.y));
+    return make_float2(h * (m.x - a.x), h * (m.y - a.y));
+}
+
+// radix 8: two radix-4 halves (even / odd inputs) joined by w8^k
+template <bool INV>
+__device__ inline void bfly8(float2 *a) {
+    float2 e[4] = {a[0], a[2], a[4], a[6]}, o[4] = {a[1], a[3], a[5], a[7]};
+    bfly<4, INV>(e);
+    bfly<4, INV>(o);
+    o[1] = mul_w8<1, INV>(o[1]);
+    o[2] = mul_w8<2, INV>(o[2]);
+    o[3] = mul_w8<3, INV>(o[3]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        a[k] = cadd(e[k], o[k]);
+        a[k + 4] = csub(e[k], o[k]);
+    }
+}
+
+// radix 16: four radix-4 DFTs over q = t mod 4, twiddles w16^{q k1}, radix-4 across q
+template <bool INV>
+__device__ inline void bfly16(float2 *a) {
+    // cos / sin of 2 pi k / 16, k = 0..9 (only q k1 <= 9 occurs)
+    const float C[10] = {1.f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f, 0.f,
+                         -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f, -1.f,
+                         -0.92387953251128675613f};
+    const float S[10] = {0.f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f, 1.f,
+                         0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f, 0.f,
+                         -0.38268343236508977173f};
+    float2 b[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        b[q][0] = a[q];
+        b[q][1] = a[q + 4];
+        b[q][2] = a[q + 8];
+        b[q][3] = a[q + 12];
+        bfly<4, INV>(b[q]);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        float2 c[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = q * k1;
+            const float2 v = b[q][k1];
+            if (e == 0) {
+                c[q] = v;
+            } else {
+                const float sn = INV ? S[e] : -S[e];  // w16^e = cos - i sin (forward)
+                c[q] = make_float2(v.x * C[e] - v.y * sn, v.x * sn + v.y * C[e]);
+            }
+        }
+        bfly<4, INV>(c);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) a[k1 + 4 * k2] = c[k2];
+    }
+}
+
+template <int R, bool INV>
+__device__ inline void bfly_any(float2 *a, const float2 *tw, int stepR) {
+    if constexpr (R == 8)
+        bfly8<INV>(a);
+    else if constexpr (R == 16)
+        bfly16<INV>(a);
+    else if constexpr (R <= 7)
+        bfly<R, INV>(a);
+    else
+        bfly_odd<R, INV>(a, tw, stepR);
+}
+
+// Ordering of LDS accesses among the lanes of one wave (the passes below run on one
+// column per wave, with no workgroup barrier).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A pass of radix R on one column of N <= kFFTMaxRing points, run by one wave: each lane
+// reads all its butterflies (at most MAXB) into registers, then writes them in place.
+template <int R>
+struct PassShape {
+    static constexpr int MAXB = (kFFTMaxRing + R * 64 - 1) / (R * 64);
+};
+
+template <int R, bool INV>
+__device__ void fft_pass_wave(float2 *col, const float2 *tw, int N, int l, int lane) {
+    constexpr int MAXB = PassShape<R>::MAXB;
+    const int M = N / (l * R);
+    const int nb = N / R;  // butterflies
+    const FastDiv divM(M);
+    float2 a[MAXB][R];
+#pragma unroll
+    for (int i = 0; i < MAXB; ++i) {
+        const int t = lane + 64 * i;
+        if (t < nb) {
+            const int u = divM.div(t), b = t - u * M;
+            const float2 *src = col + b + M * R * u;
+#pragma unroll
+            for (int q = 0; q < R; ++q) a[i][q] = src[M * q];
+        }
+    }
+    wave_sync();
+    const int ls = M * l;  // output stride between s
+    const int stepR = N / R;
+#pragma unroll
+    for (int i = 0; i < MAXB; ++i) {
+        const int t = lane + 64 * i;
+        if (t >= nb) continue;
+        const int u = divM.div(t), b = t - u * M;
+        if (u > 0) {
+#pragma unroll
+            for (int q = 1; q < R; ++q) a[i][q] = cmul(a[i][q], twid<INV>(tw, q * u * M));
+        }
+        bfly_any<R, INV>(a[i], tw, stepR);
+        float2 *dst = col + b + M * u;
+#pragma unroll
+        for (int s = 0; s < R; ++s) dst[ls * s] = a[i][s];
+    }
+    wave_sync();
+}
+
+// First pass (l = 1) of an odd prime radix r >= 17 as two small GEMMs on MFMA, in place.
+// With the inputs x_q = buf[b + M q] of butterfly b, a_t = x_t + x_{r-t},
+// b_t = x_t - x_{r-t} (t = 1..h, h = (r-1)/2):
+//   T1[s] = sum_t cos(2 pi s t / r) a_t   (s = 0..h),   T2[s] = sum_t sin(2 pi s t / r) b_t
+//   y_0 = x_0 + T1[0],  y_s = x_0 + T1[s] -+ i T2[s],  y_{r-s} = x_0 + T1[s] +- i T2[s]
+// (upper signs forward), y_s -> buf[b + M s].  GEMM rows = s (M-tiles of 16), columns =
+// (butterfly, re/im) (N-tiles of 16), K = t (steps of 4); tiles go round-robin to the
+// waves.  The cos/sin
+// operand is read from a per-prime table in MFMA-fragment order (one float2 per lane and
+// K-step, loaded one step ahead).  Butterflies read and write only their own r slots, so
+// the N-tiles are processed in rounds: all tiles of a round are computed before any of
+// their outputs is written.
+template <bool INV>
+__device__ void fft_pass_mfma_first(float2 *buf, const float2 *__restrict__ cstab, int N, int r, int ncol,
+                                    int tid) {
+    const int M = N / r;  // butterflies per column
+    const int h = (r - 1) >> 1;
+    const int nseq = ncol * M;
+    const FastDiv divH(h), divM(M);
+    for (int it = tid; it < nseq * h; it += kFFTThreads) {
+        const int sq = divH.div(it), t = it - sq * h + 1;
+        const int col = divM.div(sq), b = sq - col * M;
+        float2 *x = buf + col * N + b;
+        const float2 xt = x[M * t], xr = x[M * (r - t)];
+        x[M * t] = cadd(xt, xr);
+        x[M * (r - t)] = csub(xt, xr);
+    }
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6, row = lane & 15, grp = lane >> 4;
+    constexpr int NW = kFFTThreads / 64;
+    constexpr int TPW = kMfmaTilesPerWave;
+    const int Mt = (h + 16) >> 4;         // rows 0..h
+    const int Nt = (2 * nseq + 15) >> 4;  // real columns
+    const int Kt = (h + 3) >> 2;
+    const int NR = max(1, (TPW * NW) / Mt);  // column tiles per round
+    for (int nt0 = 0; nt0 < Nt; nt0 += NR) {
+        const int ntiles = Mt * min(NR, Nt - nt0);
+        const int nq = ntiles > wave ? (ntiles - wave + NW - 1) / NW : 0;  // tiles of this wave
+        f4 t1[TPW], t2[TPW];
+        int xo[TPW], ao[TPW];  // float offset of the column's input in buf, cos/sin row offset
+        bool colok[TPW];
+        float2 x0[TPW], anext[TPW];
+#pragma unroll
+        for (int q = 0; q < TPW; ++q) {
+            const int tile = wave + q * NW;
+            const int mt = tile % Mt, nt = nt0 + tile / Mt;
+            const int c = nt * 16 + row;
+            const int sq = c >> 1;
+            colok[q] = q < nq && sq < nseq;
+            int base = 0;
+            x0[q] = make_float2(0.f, 0.f);
+            if (colok[q]) {
+                const int col = divM.div(sq);
+                base = col * N + (sq - col * M);
+                x0[q] = buf[base];
+            }
+            xo[q] = 2 * base + (c & 1);
+            t1[q] = f4{0.f, 0.f, 0.f, 0.f};
+            t2[q] = f4{0.f, 0.f, 0.f, 0.f};
+            ao[q] = mt * Kt * 64 + lane;
+            anext[q] = q < nq ? cstab[ao[q]] : make_float2(0.f, 0.f);
+        }
+        const float *bf = reinterpret_cast<const float *>(buf);
+        for (int ks = 0; ks < Kt; ++ks) {
+            const int t = ks * 4 + grp + 1;
+            const bool tok = t <= h;
+#pragma unroll
+            for (int q = 0; q < TPW; ++q) {
+                if (q >= nq) break;  // uniform per wave
+                const float2 a = anext[q];
+                if (ks + 1 < Kt) anext[q] = cstab[ao[q] + (ks + 1) * 64];
+                const float av = (tok && colok[q]) ? bf[xo[q] + 2 * M * t] : 0.f;
+                const float bv = (tok && colok[q]) ? bf[xo[q] + 2 * M * (r - t)] : 0.f;
+                t1[q] = mfma_f32(a.x, av, t1[q]);
+                t2[q] = mfma_f32(a.y, bv, t2[q]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < TPW; ++q) {
+            if (q >= nq) break;  // uniform per wave
+            const int tile = wave + q * NW;
+            const int mt = tile % Mt, nt = nt0 + tile / Mt;
+            const int c = nt * 16 + row;
+            const int sq = c >> 1, reim = c & 1;
+            int ob = 0;
+            if (colok[q]) {
+                const int col = divM.div(sq);
+                ob = col * N + (sq - col * M);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float a1 = t1[q][i], a2 = t2[q][i];
+                const float p1 = __shfl_xor(a1, 1), p2 = __shfl_xor(a2, 1);
+                const int s = mt * 16 + 4 * grp + i;
+                if (!colok[q] || s > h) continue;
+                const float T1re = reim ? p1 : a1, T1im = reim ? a1 : p1;
+                const float T2re = reim ? p2 : a2, T2im = reim ? a2 : p2;
+                const float sg = INV ? -1.f : 1.f;
+                const float2 z = x0[q];
+                if (s == 0) {
+                    if (reim == 0) buf[ob] = make_float2(z.x + T1re, z.y + T1im);
+                } else if (reim == 0) {
+                    buf[ob + M * s] = make_float2(z.x + T1re + sg * T2im, z.y + T1im - sg * T2re);
+                } else {
+                    buf[ob + M * (r - s)] = make_float2(z.x + T1re - sg * T2im, z.y + T1im + sg * T2re);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Runs all passes in place on buf (ncol columns of N points).  A first-pass MFMA radix
+// runs across the workgroup; every other pass runs per column, column c on wave c % 4.
+template <bool INV>
+__device__ void fft_run(float2 *buf, const float2 *tw, const float2 *cstab, const int *fac, int nf, int N, int ncol,
+                        int tid) {
+    int l = 1, f = 0;
+    if (nf > 0 && fac[0] >= kMfmaMinRadix) {
+        fft_pass_mfma_first<INV>(buf, cstab, N, fac[0], ncol, tid);
+        l = fac[0];
+        f = 1;
+    }
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int c = wave; c < ncol; c += kFFTThreads / 64) {
+        float2 *col = buf + c * N;
+        int ll = l;
+        for (int g = f; g < nf; ++g) {
+            const int r = fac[g];
+            switch (r) {
+                case 8: fft_pass_wave<8, INV>(col, tw, N, ll, lane); break;
+                case 2: fft_pass_wave<2, INV>(col, tw, N, ll, lane); break;
+                case 3: fft_pass_wave<3, INV>(col, tw, N, ll, lane); break;
+                case 4: fft_pass_wave<4, INV>(col, tw, N, ll, lane); break;
+                case 5: fft_pass_wave<5, INV>(col, tw, N, ll, lane); break;
+                case 7: fft_pass_wave<7, INV>(col, tw, N, ll, lane); break;
+                case 11: fft_pass_wave<11, INV>(col, tw, N, ll, lane); break;
+                case 13: fft_pass_wave<13, INV>(col, tw, N, ll, lane); break;
+                case 17: fft_pass_wave<17, INV>(col, tw, N, ll, lane); break;
+                default: fft_pass_wave<19, INV>(col, tw, N, ll, lane); break;
+            }
+            ll *= r;
+        }
+    }
+}
+
+// Twiddles of ring j into LDS with all loads of a thread in flight together.
+__device__ inline void load_twiddles(const DevPlan &P, int j, int N, float2 *tw, int tid) {
+    const float2 *gtw = reinterpret_cast<const float2 *>(P.twiddle) + P.tw_off[j];
+    constexpr int U = 4;
+    for (int i0 = 0; i0 < N; i0 += U * kFFTThreads) {
+        float2 v[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int i = i0 + q * kFFTThreads + tid;
+            if (i < N) v[q] = gtw[i];
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int i = i0 + q * kFFTThreads + tid;
+            if (i < N) tw[i] = v[q];
+        }
+    }
+}
+
+// spectral -> grid, Fourier stage (fourier.jl:215-250): per ring, brfft of g_N and g_S as
+// one complex inverse FFT of Z = g_N + i g_S (Hermitian-extended; Im of the DC and
+// Nyquist bins ignored as FFTW's c2r does; orders above mt are zero).
+template <bool BATCH>
+__global__ __launch_bounds__(kFFTThreads, 5) void fft_synthesis_kernel(const DevPlan *__restrict__ plans,
+                                                                    CallArgs single,
+                                                                    const CallArgs *__restrict__ calls,
+                                                                    const Tile *__restrict__ tiles) {
+    extern __shared__ float2 smem[];
+    const int tid = threadIdx.x;
+    const Tile T = BATCH ? tiles[blockIdx.x] : Tile{0, 0, 0, 0};
+    const CallArgs C = BATCH ? calls[T.call] : single;
+    const DevPlan &P = plans[C.plan];
+    const float4 *__restrict__ fbuf = static_cast<const float4 *>(C.fbuf);
+    float *__restrict__ field = static_cast<float *>(C.out);
+    const long long ld_grid = C.ld_out;
+    const int K = C.K;
+    const int j = BATCH ? T.m : P.ring_order[blockIdx.x];
+    const int N = P.nlon[j], mt = P.mt[j];
+    const int k0 = (BATCH ? T.blk : (int)blockIdx.y) * P.kg;
+    const int ncol = min(P.kg, K - k0);
+    float2 *tw = smem;
+    float2 *b0 = smem + N;
+    load_twiddles(P, j, N, tw, tid);
+    const long long fb0 = P.fboff[j];
+    const int half = N / 2;
+    const int items = ncol * (half + 1);
+    const FastDiv divC(ncol);
+    constexpr int U = 8;
+    for (int i0 = 0; i0 < items; i0 += U * kFFTThreads) {
+        float4 v[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int it = i0 + q * kFFTThreads + tid;
+            v[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (it < items) {
+                const int mm = divC.div(it), col = it - mm * ncol;
+                if (mm <= mt) v[q] = fbuf[(fb0 + mm) * (long long)K + k0 + col];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int it = i0 + q * kFFTThreads + tid;
+            if (it < items) {
+                const int mm = divC.div(it), col = it - mm * ncol;
+                float2 *z = b0 + col * N;
+                const float4 w = v[q];
+                if (mm == 0) {
+                    z[0] = make_float2(w.x, w.z);
+                } else if (mm < half) {
+                    z[mm] = make_float2(w.x - w.w, w.y + w.z);
+                    z[N - mm] = make_float2(w.x + w.w, w.z - w.y);
+                } else {
+                    z[half] = make_float2(w.x, w.z);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    fft_run<true>(b0, tw, P.cstab + P.cs_off[j], P.fac + j * kMaxFactors, P.nfac[j], N, ncol, tid);
+    const float2 *res = b0;
+    const long long rn = P.ring_n[j], rs = P.ring_s[j];
+    const int lane = tid & 63;
+    for (int col = tid >> 6; col < ncol; col += kFFTThreads / 64) {  // the wave's own columns
+        float *f = field + (long long)(k0 + col) * ld_grid;
+        for (int i = lane; i < N; i += 64) {
+            const float2 z = res[col * N + i];
+            f[rn + i] = z.x;
+            f[rs + i] = z.y;
+        }
+    }
+}
+
+// grid -> spectral, Fourier stage (fourier.jl:129-166): per ring, rfft of the northern and
+// southern ring as one complex FFT of z = f_N + i f_S, separated by symmetry into f_N, f_S
+// and stored as (f_N + f_S, f_N - f_S) for orders 0..mt.
+template <bool BATCH>
+__global__ __launch_bounds__(kFFTThreads, 5) void fft_analysis_kernel(const DevPlan *__restrict__ plans,
+                                                                   CallArgs single,
+                                                                   const CallArgs *__restrict__ calls,
+                                                                   const Tile *__restrict__ tiles) {
+    extern __shared__ float2 smem[];
+    const int tid = threadIdx.x;
+    const Tile T = BATCH ? tiles[blockIdx.x] : Tile{0, 0, 0, 0};
+    const CallArgs C = BATCH ? calls[T.call] : single;
+    const DevPlan &P = plans[C.plan];
+    const float *__restrict__ field = static_cast<const float *>(C.in);
+    const long long ld_grid = C.ld_in;
+    float4 *__restrict__ fbuf = static_cast<float4 *>(C.fbuf);
+    const int K = C.K;
+    const int j = BATCH ? T.m : P.ring_order[blockIdx.x];
+    const int N = P.nlon[j], mt = P.mt[j];
+    const int k0 = (BATCH ? T.blk : (int)blockIdx.y) * P.kg;
+    const int ncol = min(P.kg, K - k0);
+    float2 *tw = smem;
+    float2 *b0 = smem + N;
+    load_twiddles(P, j, N, tw, tid);
+    const long long rn = P.ring_n[j], rs = P.ring_s[j];
+    const int items = ncol * N;
+    const FastDiv divN(N);
+    constexpr int U = 8;
+    for (int i0 = 0; i0 < items; i0 += U * kFFTThreads) {
+        float2 v[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int it = i0 + q * kFFTThreads + tid;
+            if (it < items) {
+                const int col = divN.div(it), i = it - col * N;
+                const float *f = field + (long long)(k0 + col) * ld_grid;
+                v[q] = make_float2(f[rn + i], f[rs + i]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int it = i0 + q * kFFTThreads + tid;
+            if (it < items) b0[it] = v[q];
+        }
+    }
+    __syncthreads();
+    fft_run<false>(b0, tw, P.cstab + P.cs_off[j], P.fac + j * kMaxFactors, P.nfac[j], N, ncol, tid);
+    __syncthreads();
+    const float2 *res = b0;
+    const long long fb0 = P.fboff[j];
+    const int outs = ncol * (mt + 1);
+    const FastDiv divC(ncol);
+    for (int it = tid; it < outs; it += kFFTThreads) {
+        const int mm = divC.div(it), col = it - mm * ncol;
+        const float2 p = res[col * N + mm];
+        const float2 q = res[col * N + (mm == 0 ? 0 : N - mm)];
+        // f_N = (Z_m + conj Z_{N-m})/2, f_S = (Z_m - conj Z_{N-m})/(2i)
+        const float nre = 0.5f * (p.x + q.x), nim = 0.5f * (p.y - q.y);
+        const float sre = 0.5f * (p.y + q.y), sim = 0.5f * (q.x - p.x);
+        fbuf[(fb0 + mm) * (long long)K + k0 + col] = make_float4(nre + sre, nim + sim, nre - sre, nim - sim);
+    }
+}
+
+// ================================================================== entry points
+
+const void *fft_kernel(int stage, bool batch) {
+    if (stage == kFFTSynthesis)
+        return batch ? reinterpret_cast<const void *>(&fft_synthesis_kernel<true>)
+                     : reinterpret_cast<const void *>(&fft_synthesis_kernel<false>);
+    return batch ? reinterpret_cast<const void *>(&fft_analysis_kernel<true>)
+                 : reinterpret_cast<const void *>(&fft_analysis_kernel<false>);
+}
+
+}  // namespace sht

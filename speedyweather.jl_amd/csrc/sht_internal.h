@@ -2,6 +2,8 @@
 #pragma once
 #include <stdint.h>
 
+#include <string>
+
 namespace sht {
 
 // Column group: 32 complex columns = 64 real MFMA columns per Legendre workgroup.
@@ -16,13 +18,45 @@ constexpr int kDegPerWG = 64;
 constexpr int kChunk = 32;
 constexpr int kMaxFactors = 24;
 constexpr int kFFTThreads = 256;
+// Largest ring the FFT kernel takes, and columns per FFT workgroup (one wave each).
+constexpr int kFFTMaxRing = 1024;
+constexpr int kFFTCols = 4;
+// First-pass prime radices from this size up run on MFMA; smaller radices (and any
+// later pass) run as register butterflies, which exist for 2, 3, 4, 5, 7, 11, 13, 17, 19.
+constexpr int kMfmaMinRadix = 17;
+constexpr int kMaxRegisterRadix = 19;
+// MFMA GEMM pass of the FFT: tiles (16 output frequencies x 16 real columns) a wave keeps
+// in flight within one round; rows up to 16 * kMfmaTilesPerWave * waves, i.e. radices up
+// to kMaxMfmaRadix (255).
+constexpr int kMfmaTilesPerWave = 2;
+constexpr int kMaxMfmaRadix = 2 * 16 * kMfmaTilesPerWave * (kFFTThreads / 64) - 1;
 
 enum Mode { kFP32 = 0, kFP16x3 = 1, kBF16x3 = 2 };
 
-struct LegTile {
-    int m;    // order (0-based)
-    int blk;  // ring group (synthesis) or degree group (analysis)
+// One workgroup's work item.  Legendre stages: order m, ring group (synthesis) or degree
+// group (analysis), column group cg.  FFT stages: m = ring, blk = column group.  call
+// indexes the launch's call arguments (always 0 for a single transform).
+struct Tile {
+    int call;
+    int m;
+    int blk;
+    int cg;
 };
+
+// Arguments of one transform call inside a launch.
+struct CallArgs {
+    int plan;             // index into the launch's plan array
+    int K;                // columns
+    int unscale;          // synthesis: divide by cos(lat)
+    int pad;
+    const void *in;       // synthesis: coefficients; analysis: grid
+    void *out;            // synthesis: grid; analysis: coefficients
+    long long ld_in;      // column strides (complex elements for spectra, floats for grids)
+    long long ld_out;
+    void *fbuf;           // Fourier scratch of this call
+};
+
+enum Stage { kLegendreSynthesis = 0, kFFTSynthesis = 1, kFFTAnalysis = 2, kLegendreAnalysis = 3 };
 
 // Device view of a plan, passed by value to every kernel.
 struct DevPlan {
@@ -66,6 +100,11 @@ struct DevPlan {
     const long long *tw_off;
     const int *fac;        // kMaxFactors radices per ring
     const int *nfac;
+    // cos/sin operand of the MFMA radix pass, per prime in fragment order
+    // [row tile][K-step][lane] = (cos, sin)(2 pi s t / r), s = 16 tile + lane%16,
+    // t = 4 step + lane/16 + 1 (zero outside s, t <= (r-1)/2); ring j's table at cs_off[j]
+    const float2 *cstab;
+    const long long *cs_off;
 
     // UV_from_vordiv factors (per packed lm): x = -m/(l(l+1)), u1 = eps_lm / l,
     // u2 = eps_{l+1,m} / (l+1) (0 on the last row)
@@ -75,18 +114,16 @@ struct DevPlan {
     const int *l_of;       // 0-based degree of packed index
 };
 
-// Kernel launchers (sht_kernels.hip).
-int launch_legendre_synthesis(const DevPlan &P, int mode, const LegTile *tiles, int ntiles,
-                              const void *coeffs, long long ld_spec, void *fbuf, int K,
-                              int unscale, void *stream);
-int launch_legendre_analysis(const DevPlan &P, int mode, const LegTile *tiles, int ntiles,
-                             const void *fbuf, void *coeffs, long long ld_spec, int K,
-                             void *stream);
-int launch_fft_synthesis(const DevPlan &P, const void *fbuf, void *field, long long ld_grid,
-                         int K, void *stream);
-int launch_fft_analysis(const DevPlan &P, const void *field, long long ld_grid, void *fbuf,
-                        int K, void *stream);
-int launch_uv_from_vordiv(const DevPlan &P, const void *vor, const void *div, void *U,
+// Records the message sht_last_error() returns on this thread.
+void set_last_error(const std::string &msg);
+
+// Kernel entry points.  Every stage kernel takes
+//   (const DevPlan *plans, CallArgs single, const CallArgs *calls, const Tile *tiles)
+// and, for batch = false, reads the single call's arguments by value and its column group
+// from blockIdx.y; for batch = true, the call and column group of each tile.
+const void *legendre_kernel(int stage, int mode, bool batch);
+const void *fft_kernel(int stage, bool batch);
+int launch_uv_from_vordiv(const DevPlan *P, int nharm, const void *vor, const void *div, void *U,
                           void *V, long long ld, int K, float radius, void *stream);
 
 }  // namespace sht

@@ -1,0 +1,410 @@
+// gfx950 Legendre-stage kernels of the spherical harmonic transform.
+//
+// For every order m a dense contraction between the precomputed Legendre polynomials
+// Λ_m (degrees x rings) and the K spectral / Fourier columns, split into even and odd
+// degrees l-m so one contraction serves both hemispheres (the even/odd symmetry of
+// SpeedyTransforms/src/legendre.jl:2-33,104-126).  Runs on MFMA: v_mfma_f32_16x16x4_f32
+// in FP32 mode; v_mfma_f32_16x16x32_{f16,bf16} with a hi/lo split of both operands (three
+// products, FP32 accumulation) in the split modes.
+//
+// The contraction dimension is streamed in chunks of 32 (degrees per parity, or rings):
+// the next chunk's operands are loaded into registers while the current chunk's MFMAs
+// run, and the staged operand tile in LDS is double-buffered (one barrier per chunk).
+//
+// Fourier buffer (owned by the plan): row (fboff[j] + m) holds, for every column k, a
+// float4 (N.re, N.im, S.re, S.im).  Synthesis writes the northern/southern Legendre sums
+// g_N, g_S there (legendre.jl:84-91); analysis reads the even/odd combinations
+// f_N + f_S, f_N - f_S (legendre.jl:176-183, the solid angle folded into Λ).
+#include "sht_device.h"
+
+namespace sht {
+
+typedef u16 u16x4 __attribute__((ext_vector_type(4)));
+
+// LDS tiles of one chunk: [buffer][parity][real column][contraction index], padded rows
+constexpr int kRowF = kChunk + 4;  // FP32 row (floats): 16-byte aligned, bank-spread
+constexpr int kRowH = kChunk + 8;  // 16-bit row: 16-byte aligned, bank-spread
+constexpr int kTileF = 2 * kColsReal * kRowF;
+constexpr int kTileH = 2 * kColsReal * kRowH;
+
+template <int MODE>
+struct LegSmem {
+    static constexpr int bytes = MODE == kFP32 ? 2 * kTileF * 4 : 2 * 2 * kTileH * 2 + 2 * 2 * kColsReal * 4;
+};
+
+// Stages 4 contraction values x (re, im) of one complex column (real columns 2kk, 2kk+1)
+// and parity p into the chunk tile: FP32 as is; split modes as hi/lo with one power-of-two
+// scale per (parity, real column) over the 32 values the 8 lanes of a column group hold.
+template <int MODE>
+__device__ inline void stage4(unsigned char *sm, int buf, int p, int kk, int g, const float (&re)[4],
+                              const float (&im)[4]) {
+    if constexpr (MODE == kFP32) {
+        float *B = reinterpret_cast<float *>(sm) + buf * kTileF + p * kColsReal * kRowF;
+        *reinterpret_cast<float4 *>(B + (2 * kk) * kRowF + 4 * g) = make_float4(re[0], re[1], re[2], re[3]);
+        *reinterpret_cast<float4 *>(B + (2 * kk + 1) * kRowF + 4 * g) = make_float4(im[0], im[1], im[2], im[3]);
+    } else {
+        typedef typename Split<MODE>::T T16;
+        u16 *Bh = reinterpret_cast<u16 *>(sm) + (buf * 2 + 0) * kTileH + p * kColsReal * kRowH;
+        u16 *Bl = reinterpret_cast<u16 *>(sm) + (buf * 2 + 1) * kTileH + p * kColsReal * kRowH;
+        float *Bs = reinterpret_cast<float *>(reinterpret_cast<u16 *>(sm) + 4 * kTileH) + (buf * 2 + p) * kColsReal;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const float *x = c == 0 ? re : im;
+            float s = 1.f, sinv = 1.f;
+            if constexpr (MODE == kFP16x3) {
+                float amax = fmaxf(fmaxf(fabsf(x[0]), fabsf(x[1])), fmaxf(fabsf(x[2]), fabsf(x[3])));
+                amax = fmaxf(amax, __shfl_xor(amax, 1));
+                amax = fmaxf(amax, __shfl_xor(amax, 2));
+                amax = fmaxf(amax, __shfl_xor(amax, 4));
+                fp16_scale(amax, s, sinv);
+            }
+            u16x4 hi, lo;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float v = x[e] * s;
+                const T16 h = (T16)v;
+                const T16 l = (T16)(v - (float)h);
+                hi[e] = __builtin_bit_cast(u16, h);
+                lo[e] = __builtin_bit_cast(u16, l);
+            }
+            const int col = 2 * kk + c;
+            *reinterpret_cast<u16x4 *>(Bh + col * kRowH + 4 * g) = hi;
+            *reinterpret_cast<u16x4 *>(Bl + col * kRowH + 4 * g) = lo;
+            if (g == 0) Bs[col] = sinv;
+        }
+    }
+}
+
+// acc += A (16 rows x 32 contraction, this lane's fragment) x B tile columns cb*16..cb*16+15
+// of parity p.  FP32: af[8] holds the lane's A values for the 8 K-steps of 4.
+template <int MODE, int NCB>
+__device__ inline void mma_chunk(const unsigned char *sm, int buf, int p, int row, int grp, const float (&af)[8],
+                                 typename Split<MODE == kFP32 ? kFP16x3 : MODE>::V ah,
+                                 typename Split<MODE == kFP32 ? kFP16x3 : MODE>::V al, f4 (&acc)[NCB]) {
+    if constexpr (MODE == kFP32) {
+        const float *B = reinterpret_cast<const float *>(sm) + buf * kTileF + p * kColsReal * kRowF;
+#pragma unroll
+        for (int ks = 0; ks < kChunk / 4; ++ks) {
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) {
+                const float b = B[(cb * 16 + row) * kRowF + 4 * ks + grp];
+                acc[cb] = mfma_f32(af[ks], b, acc[cb]);
+            }
+        }
+    } else {
+        typedef typename Split<MODE>::V V8;
+        const u16 *Bh = reinterpret_cast<const u16 *>(sm) + (buf * 2 + 0) * kTileH + p * kColsReal * kRowH;
+        const u16 *Bl = reinterpret_cast<const u16 *>(sm) + (buf * 2 + 1) * kTileH + p * kColsReal * kRowH;
+        const float *Bs =
+            reinterpret_cast<const float *>(reinterpret_cast<const u16 *>(sm) + 4 * kTileH) + (buf * 2 + p) * kColsReal;
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) {
+            const int col = cb * 16 + row;
+            const V8 bh = __builtin_bit_cast(V8, *reinterpret_cast<const u16x8 *>(Bh + col * kRowH + 8 * grp));
+            const V8 bl = __builtin_bit_cast(V8, *reinterpret_cast<const u16x8 *>(Bl + col * kRowH + 8 * grp));
+            if constexpr (MODE == kFP16x3) {
+                f4 t = Split<MODE>::mma(al, bh, f4{0.f, 0.f, 0.f, 0.f});
+                t = Split<MODE>::mma(ah, bl, t);
+                t = Split<MODE>::mma(ah, bh, t);
+                acc[cb] += t * Bs[col];
+            } else {
+                acc[cb] = Split<MODE>::mma(al, bh, acc[cb]);
+                acc[cb] = Split<MODE>::mma(ah, bl, acc[cb]);
+                acc[cb] = Split<MODE>::mma(ah, bh, acc[cb]);
+            }
+        }
+    }
+}
+
+template <int MODE>
+using V8of = typename Split<MODE == kFP32 ? kFP16x3 : MODE>::V;
+
+// ============================================================ Legendre synthesis
+//
+// Workgroup = (order m, group of 64 rings, 32 complex columns).  Wave w owns ring tile
+// 4*blk+w (16 rings) and all 64 real columns, for both parities: 8 accumulators.
+// Degrees l = m + 2l' + p stream in chunks of 32 l' per parity; a staging thread loads 8
+// consecutive degrees of one complex column (kk = tid/8, degrees 8(tid%8)..+7).
+
+template <int MODE, bool BATCH>
+__global__ __launch_bounds__(256, 4) void legendre_synthesis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
+                                                                 const CallArgs *__restrict__ calls,
+                                                                 const Tile *__restrict__ tiles) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const Tile T = tiles[blockIdx.x];
+    const CallArgs C = BATCH ? calls[T.call] : single;
+    const DevPlan &P = plans[C.plan];
+    const int m = T.m, cg = BATCH ? T.cg : (int)blockIdx.y;
+    const float2 *__restrict__ coeffs = static_cast<const float2 *>(C.in);
+    const long long ld_spec = C.ld_in;
+    float4 *__restrict__ fbuf = static_cast<float4 *>(C.fbuf);
+    const int K = C.K, unscale = C.unscale;
+    const int n = P.n;
+    const int j0 = P.j0[m];
+    const int J = n - j0;
+    const int L = P.lmax - m;
+    const int Le = (L + 1) >> 1;
+    const int nch = (Le + kChunk - 1) / kChunk;
+    const int Jt = (J + 15) >> 4;
+    const int tile = T.blk * 4 + w;
+    const bool active = tile < Jt;
+    const long long off = P.lm_off[m];
+    const int kbase = cg * kColsComplex;
+    const int row = lane & 15, grp = lane >> 4;
+    // staging role
+    const int kk = tid >> 3, g = tid & 7;
+    const bool kok = kbase + kk < K;
+    const float2 *csrc = coeffs + (long long)(kok ? kbase + kk : 0) * ld_spec + off;
+
+    __shared__ alignas(16) unsigned char sm[LegSmem<MODE>::bytes];
+
+    f4 acc[2][4];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) acc[p][cb] = f4{0.f, 0.f, 0.f, 0.f};
+
+    // register prefetch: 8 degrees of one column, and this wave's Λ fragments
+    float2 v[8];
+    auto load_coeffs = [&](int c) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int o = 64 * c + 8 * g + e;
+            v[e] = (kok && o < L) ? csrc[o] : make_float2(0.f, 0.f);
+        }
+    };
+    float af[2][8], afn[2][8];
+    V8of<MODE> ah[2], al[2], ahn[2], aln[2];
+    auto load_lambda = [&](int c, float (&f)[2][8], V8of<MODE> (&h)[2], V8of<MODE> (&lo)[2]) {
+        if (!active) return;
+        if constexpr (MODE == kFP32) {
+            const float *A = P.syn_f32 + P.syn_off[m];
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const float *Ap = A + ((long long)(p * Jt + tile) * (nch * kChunk) + c * kChunk) * 16;
+#pragma unroll
+                for (int ks = 0; ks < kChunk / 4; ++ks) f[p][ks] = Ap[(ks * 4 + grp) * 16 + row];
+            }
+        } else {
+            const V8of<MODE> *Ah = reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.syn_hi) + P.syn_off[m]);
+            const V8of<MODE> *Al = reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.syn_lo) + P.syn_off[m]);
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const long long fi = ((long long)(p * Jt + tile) * nch + c) * 64 + lane;
+                h[p] = Ah[fi];
+                lo[p] = Al[fi];
+            }
+        }
+    };
+
+    load_coeffs(0);
+    load_lambda(0, afn, ahn, aln);
+    for (int c = 0; c < nch; ++c) {
+        const int buf = c & 1;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            float re[4], im[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                re[i] = v[2 * i + p].x;
+                im[i] = v[2 * i + p].y;
+            }
+            stage4<MODE>(sm, buf, p, kk, g, re, im);
+        }
+        if (c + 1 < nch) load_coeffs(c + 1);
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+#pragma unroll
+            for (int ks = 0; ks < 8; ++ks) af[p][ks] = afn[p][ks];
+            ah[p] = ahn[p];
+            al[p] = aln[p];
+        }
+        if (c + 1 < nch) load_lambda(c + 1, afn, ahn, aln);
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int p = 0; p < 2; ++p) mma_chunk<MODE, 4>(sm, buf, p, row, grp, af[p], ah[p], al[p], acc[p]);
+        }
+    }
+
+    if (!active) return;  // no further barriers below
+    // epilogue: N = E + O, S = E - O (legendre.jl:84-91), optional 1/cos(lat)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        const int col = cb * 16 + row;
+        const int kc = col >> 1, reim = col & 1;
+        const int k = kbase + kc;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = 4 * grp + i;
+            const int j = j0 + 16 * tile + r;
+            float e = acc[0][cb][i], o = acc[1][cb][i];
+            if constexpr (MODE == kFP16x3) {
+                const float sa = P.syn_sinv[P.syn_soff[m] + 16 * tile + r];
+                e *= sa;
+                o *= sa;
+            }
+            float nv = e + o, sv = e - o;
+            if (unscale && j < n) {
+                const float ci = P.coslat_inv[j];
+                nv *= ci;
+                sv *= ci;
+            }
+            const float n_other = __shfl_xor(nv, 1);
+            const float s_other = __shfl_xor(sv, 1);
+            if (j < n && k < K) {
+                float *dst = reinterpret_cast<float *>(fbuf + (P.fboff[j] + m) * (long long)K + k);
+                if (reim == 0) {
+                    *reinterpret_cast<float2 *>(dst) = make_float2(nv, n_other);
+                } else {
+                    *reinterpret_cast<float2 *>(dst + 2) = make_float2(s_other, sv);
+                }
+            }
+        }
+    }
+}
+
+// ============================================================== Legendre analysis
+//
+// Workgroup = (order m, group of 64 degrees, 32 complex columns).  Wave w owns parity
+// p = w&1 and degree tile lt = 2*blk + (w>>1) (16 rows of l' with l = m + 2l' + p) for
+// all 64 real columns.  Rings stream in chunks of 32; a staging thread loads 4 consecutive
+// rings of one complex column (kk = tid/8, rings 4(tid%8)..+3).  Every output coefficient
+// is written exactly once (no atomics, deterministic).
+
+template <int MODE, bool BATCH>
+__global__ __launch_bounds__(256, 4) void legendre_analysis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
+                                                                const CallArgs *__restrict__ calls,
+                                                                const Tile *__restrict__ tiles) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const Tile T = tiles[blockIdx.x];
+    const CallArgs C = BATCH ? calls[T.call] : single;
+    const DevPlan &P = plans[C.plan];
+    const int m = T.m, cg = BATCH ? T.cg : (int)blockIdx.y;
+    const float4 *__restrict__ fbuf = static_cast<const float4 *>(C.fbuf);
+    float2 *__restrict__ coeffs = static_cast<float2 *>(C.out);
+    const long long ld_spec = C.ld_out;
+    const int K = C.K;
+    const int n = P.n;
+    const int j0 = P.j0[m];
+    const int J = n - j0;
+    const int L = P.lmax - m;
+    const int NG = (L + kDegPerWG - 1) / kDegPerWG;
+    const int nt = 2 * NG;
+    const int JC = (J + kChunk - 1) / kChunk;
+    const int p = w & 1;
+    const int lt = 2 * T.blk + (w >> 1);
+    const long long off = P.lm_off[m];
+    const int kbase = cg * kColsComplex;
+    const int row = lane & 15, grp = lane >> 4;
+    const int kk = tid >> 3, g = tid & 7;
+    const bool kok = kbase + kk < K;
+    const int kgl = kok ? kbase + kk : 0;
+
+    constexpr int OUTB = kColsComplex * (kDegPerWG + 1) * 8;
+    constexpr int SMB = LegSmem<MODE>::bytes > OUTB ? LegSmem<MODE>::bytes : OUTB;
+    __shared__ alignas(16) unsigned char sm[SMB];
+
+    f4 acc[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) acc[cb] = f4{0.f, 0.f, 0.f, 0.f};
+
+    float4 v[4];
+    auto load_fourier = [&](int c) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int j = j0 + kChunk * c + 4 * g + i;
+            v[i] = (kok && j < n) ? fbuf[(P.fboff[j] + m) * (long long)K + kgl] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    };
+    float af[8], afn[8];
+    V8of<MODE> ah, al, ahn, aln;
+    auto load_lambda = [&](int c, float (&f)[8], V8of<MODE> &h, V8of<MODE> &lo) {
+        if constexpr (MODE == kFP32) {
+            const float *A = P.ana_f32 + P.ana_off[m] + ((long long)(p * nt + lt) * (JC * kChunk) + c * kChunk) * 16;
+#pragma unroll
+            for (int ks = 0; ks < kChunk / 4; ++ks) f[ks] = A[(ks * 4 + grp) * 16 + row];
+        } else {
+            const V8of<MODE> *Ah = reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_hi) + P.ana_off[m]);
+            const V8of<MODE> *Al = reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_lo) + P.ana_off[m]);
+            const long long fi = ((long long)(p * nt + lt) * JC + c) * 64 + lane;
+            h = Ah[fi];
+            lo = Al[fi];
+        }
+    };
+
+    if (JC > 0) {
+        load_fourier(0);
+        load_lambda(0, afn, ahn, aln);
+    }
+    for (int c = 0; c < JC; ++c) {
+        const int buf = c & 1;
+        {
+            float re[4], im[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                re[i] = v[i].x;
+                im[i] = v[i].y;
+            }
+            stage4<MODE>(sm, buf, 0, kk, g, re, im);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                re[i] = v[i].z;
+                im[i] = v[i].w;
+            }
+            stage4<MODE>(sm, buf, 1, kk, g, re, im);
+        }
+        if (c + 1 < JC) load_fourier(c + 1);
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) af[ks] = afn[ks];
+        ah = ahn;
+        al = aln;
+        if (c + 1 < JC) load_lambda(c + 1, afn, ahn, aln);
+        __syncthreads();
+        mma_chunk<MODE, 4>(sm, buf, p, row, grp, af, ah, al, acc);
+    }
+    __syncthreads();
+    // stage the 64-degree x 32-column output tile, then write l-contiguous columns
+    float2 *Out = reinterpret_cast<float2 *>(sm);  // [k][l - m - 64 blk], row kDegPerWG + 1
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        const int col = cb * 16 + row;
+        const int kc = col >> 1, reim = col & 1;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = 4 * grp + i;
+            const int lq = 16 * (w >> 1) + r;  // l' - 32 blk
+            float val = acc[cb][i];
+            if constexpr (MODE == kFP16x3) val *= P.ana_sinv[P.ana_soff[m] + (p * nt + lt) * 16 + r];
+            float *o = reinterpret_cast<float *>(&Out[kc * (kDegPerWG + 1) + 2 * lq + p]);
+            o[reim] = val;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        const int idx = it * 256 + tid;
+        const int kc = idx >> 6, o = idx & 63;
+        const int og = kDegPerWG * T.blk + o, kq = kbase + kc;
+        if (og < L && kq < K) coeffs[(long long)kq * ld_spec + off + og] = Out[kc * (kDegPerWG + 1) + o];
+    }
+}
+
+// ================================================================== entry points
+
+template <int MODE>
+static const void *legendre_entry(int stage, bool batch) {
+    if (stage == kLegendreSynthesis)
+        return batch ? reinterpret_cast<const void *>(&legendre_synthesis_kernel<MODE, true>)
+                     : reinterpret_cast<const void *>(&legendre_synthesis_kernel<MODE, false>);
+    return batch ? reinterpret_cast<const void *>(&legendre_analysis_kernel<MODE, true>)
+                 : reinterpret_cast<const void *>(&legendre_analysis_kernel<MODE, false>);
+}
+
+const void *legendre_kernel(int stage, int mode, bool batch) {
+    if (mode == kFP32) return legendre_entry<kFP32>(stage, batch);
+    if (mode == kFP16x3) return legendre_entry<kFP16x3>(stage, batch);
+    return legendre_entry<kBF16x3>(stage, batch);
+}
+
+}  // namespace sht

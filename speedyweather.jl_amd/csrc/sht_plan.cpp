@@ -19,8 +19,9 @@
 #include "../../include/sht.h"
 #include "sht_internal.h"
 
+using sht::CallArgs;
 using sht::DevPlan;
-using sht::LegTile;
+using sht::Tile;
 
 namespace {
 
@@ -30,6 +31,12 @@ int fail(int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
+
+}  // namespace
+
+void sht::set_last_error(const std::string &msg) { g_err = msg; }
+
+namespace {
 
 // ------------------------------------------------------------------ geometry
 
@@ -216,10 +223,15 @@ struct sht_plan {
     sht_desc d{};
     Geometry g;
     DevPlan P{};
+    const DevPlan *dP = nullptr;  // device copy of P
     std::vector<void *> allocs;
-    const LegTile *syn_tiles = nullptr;
-    const LegTile *ana_tiles = nullptr;
+    const Tile *syn_tiles = nullptr;
+    const Tile *ana_tiles = nullptr;
     int nsyn = 0, nana = 0;
+    // host copies for batches: Legendre tiles (heavy first) with their costs, FFT ring order
+    std::vector<Tile> h_syn, h_ana;
+    std::vector<double> h_syn_cost, h_ana_cost, h_ring_cost;
+    std::vector<int> h_ring_order;
     void *fbuf = nullptr;
     long long device_bytes = 0;
     Profile prof;
@@ -271,27 +283,37 @@ void free_plan(sht_plan *p) {
     delete p;
 }
 
-// Radices of the ring FFT, in pass order: odd primes >= 11 first (largest first; the
-// first pass needs no twiddles and these run as MFMA GEMMs), then 4s, 2, 3, 5, 7.
+// Radices of the ring FFT, in pass order: the largest prime >= kMfmaMinRadix first (the
+// first pass runs as an MFMA GEMM and needs no twiddles), then 8s, 4, 2, 3, 5, 7, 11, 13,
+// then any further prime >= 17.  Returns the count, or -1 if a radix has no kernel.
 int factorize(int N, int *fac) {
     int nf = 0, r = N;
     std::vector<int> big;
     int tmp = r;
-    for (int q : {2, 3, 5, 7})
+    for (int q : {2, 3, 5, 7, 11, 13})
         while (tmp % q == 0) tmp /= q;
-    for (int p = 11; p <= tmp; p += 2) {  // tmp has no factor below 11: p divides only if prime
+    for (int p = 17; p <= tmp; p += 2) {  // tmp has no factor below 17: p divides only if prime
         while (tmp % p == 0) {
             big.push_back(p);
             tmp /= p;
         }
     }
     std::sort(big.begin(), big.end(), [](int a, int b) { return a > b; });
-    for (int p : big) { fac[nf++] = p; r /= p; }
-    while (r % 4 == 0) { fac[nf++] = 4; r /= 4; }
-    const int small[4] = {2, 3, 5, 7};
-    for (int s : small)
-        while (r % s == 0) { fac[nf++] = s; r /= s; }
-    return nf;
+    if (!big.empty()) {
+        if (big[0] > sht::kMaxMfmaRadix) return -1;
+        fac[nf++] = big[0];
+        r /= big[0];
+    }
+    while (r % 8 == 0) { fac[nf++] = 8; r /= 8; }
+    if (r % 4 == 0) { fac[nf++] = 4; r /= 4; }
+    for (int q : {2, 3, 5, 7, 11, 13})
+        while (r % q == 0) { fac[nf++] = q; r /= q; }
+    for (size_t i = 1; i < big.size(); ++i) {
+        if (big[i] > sht::kMaxRegisterRadix) return -1;
+        fac[nf++] = big[i];
+        r /= big[i];
+    }
+    return r == 1 ? nf : -1;
 }
 
 int build_plan(sht_plan *pl) {
@@ -433,6 +455,8 @@ int build_plan(sht_plan *pl) {
     long long tw_total = 0;
     for (int j = 0; j < n; ++j) {
         nfac[j] = factorize(g.nlon[j], &fac[(size_t)j * sht::kMaxFactors]);
+        if (nfac[j] < 0)
+            return fail(SHT_ERR_UNSUPPORTED, "ring length " + std::to_string(g.nlon[j]) + " has a prime factor the FFT cannot run");
         tw_off[j] = tw_total;
         tw_total += g.nlon[j];
     }
@@ -445,48 +469,86 @@ int build_plan(sht_plan *pl) {
             tw[2 * (tw_off[j] + k) + 1] = (float)std::sin(a);
         }
     }
+    // cos/sin fragment tables of the first-pass MFMA primes (shared by rings with one prime)
+    std::vector<float2> cstab;
+    std::vector<long long> cs_off(n, 0);
+    {
+        std::vector<long long> prime_off(sht::kMaxMfmaRadix + 1, -1);
+        for (int j = 0; j < n; ++j) {
+            const int r = nfac[j] > 0 ? fac[(size_t)j * sht::kMaxFactors] : 1;
+            if (r < sht::kMfmaMinRadix) continue;
+            if (prime_off[r] < 0) {
+                prime_off[r] = (long long)cstab.size();
+                const int h = (r - 1) / 2, Mt = (h + 16) / 16, Kt = (h + 3) / 4;
+                for (int mt = 0; mt < Mt; ++mt)
+                    for (int ks = 0; ks < Kt; ++ks)
+                        for (int lane = 0; lane < 64; ++lane) {
+                            const int s = 16 * mt + (lane & 15), t = 4 * ks + (lane >> 4) + 1;
+                            float2 v = {0.f, 0.f};
+                            if (s <= h && t <= h) {
+                                const double a = 2.0 * M_PI * (double)((long long)s * t % r) / r;
+                                v.x = (float)std::cos(a);
+                                v.y = (float)std::sin(a);
+                            }
+                            cstab.push_back(v);
+                        }
+            }
+            cs_off[j] = prime_off[r];
+        }
+        if (cstab.empty()) cstab.push_back({0.f, 0.f});
+    }
     std::vector<double> ring_cost(n);
     for (int j = 0; j < n; ++j) {
         double c = 0;
         for (int f = 0; f < nfac[j]; ++f) {
             const int r = fac[(size_t)j * sht::kMaxFactors + f];
-            c += r <= 5 ? 1.0 : (double)r;
+            c += r <= 8 ? 1.0 : (r < sht::kMfmaMinRadix ? 0.5 * r : 0.25 * r);
         }
         ring_cost[j] = c * g.nlon[j];
     }
     std::vector<int> ring_order(n);
     std::iota(ring_order.begin(), ring_order.end(), 0);
     std::stable_sort(ring_order.begin(), ring_order.end(), [&](int a, int b) { return ring_cost[a] > ring_cost[b]; });
-    int kg = (65536 / 8 - g.nlon_max) / (2 * g.nlon_max);
-    kg = std::max(1, std::min(8, kg));
+    if (g.nlon_max > sht::kFFTMaxRing) return fail(SHT_ERR_UNSUPPORTED, "ring too long for the LDS FFT");
+    const int kg = sht::kFFTCols;
     P.kg = kg;
-    P.fft_smem = (g.nlon_max + 2 * kg * g.nlon_max) * 8;
-    if (P.fft_smem > 160 * 1024) return fail(SHT_ERR_UNSUPPORTED, "ring too long for the LDS FFT");
+    P.fft_smem = (g.nlon_max + kg * g.nlon_max) * 8;  // twiddles + one in-place buffer
 
     // --------------------------------------------------------------- tiles
-    std::vector<LegTile> syn_t, ana_t;
+    std::vector<Tile> syn_t, ana_t;
     std::vector<double> syn_c, ana_c;
     for (int m = 0; m < mmax; ++m) {
         const int J = n - g.j0[m], L = lmax - m;
         for (int b = 0; b * sht::kRingsPerWG < J; ++b) {
-            syn_t.push_back({m, b});
+            syn_t.push_back({0, m, b, 0});
             syn_c.push_back((double)std::min(sht::kRingsPerWG, J - b * sht::kRingsPerWG) * L);
         }
         for (int b = 0; b * sht::kDegPerWG < L; ++b) {
-            ana_t.push_back({m, b});
+            ana_t.push_back({0, m, b, 0});
             ana_c.push_back((double)std::min(sht::kDegPerWG, L - b * sht::kDegPerWG) * std::max(J, 1));
         }
     }
-    auto sort_tiles = [](std::vector<LegTile> &t, const std::vector<double> &c) {
+    auto sort_tiles = [](std::vector<Tile> &t, std::vector<double> &c) {
         std::vector<size_t> idx(t.size());
         std::iota(idx.begin(), idx.end(), 0);
         std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return c[a] > c[b]; });
-        std::vector<LegTile> out(t.size());
-        for (size_t i = 0; i < idx.size(); ++i) out[i] = t[idx[i]];
+        std::vector<Tile> out(t.size());
+        std::vector<double> oc(t.size());
+        for (size_t i = 0; i < idx.size(); ++i) {
+            out[i] = t[idx[i]];
+            oc[i] = c[idx[i]];
+        }
         t.swap(out);
+        c.swap(oc);
     };
     sort_tiles(syn_t, syn_c);
     sort_tiles(ana_t, ana_c);
+    pl->h_syn = syn_t;
+    pl->h_ana = ana_t;
+    pl->h_syn_cost = syn_c;
+    pl->h_ana_cost = ana_c;
+    pl->h_ring_order = ring_order;
+    pl->h_ring_cost = ring_cost;
 
     // ---------------------------------------------------- UV_from_vordiv factors
     std::vector<float> uvx(g.nharm), uv1(g.nharm), uv2(g.nharm);
@@ -537,6 +599,8 @@ int build_plan(sht_plan *pl) {
     if ((rc = dev_upload(A, tw_off, &P.tw_off))) return rc;
     if ((rc = dev_upload(A, fac, &P.fac))) return rc;
     if ((rc = dev_upload(A, nfac, &P.nfac))) return rc;
+    if ((rc = dev_upload(A, cstab, &P.cstab))) return rc;
+    if ((rc = dev_upload(A, cs_off, &P.cs_off))) return rc;
     if ((rc = dev_upload(A, uvx, &P.uv_x))) return rc;
     if ((rc = dev_upload(A, uv1, &P.uv_1))) return rc;
     if ((rc = dev_upload(A, uv2, &P.uv_2))) return rc;
@@ -545,6 +609,16 @@ int build_plan(sht_plan *pl) {
     if ((rc = dev_upload(A, ana_t, &pl->ana_tiles))) return rc;
     pl->nsyn = (int)syn_t.size();
     pl->nana = (int)ana_t.size();
+    {
+        std::vector<DevPlan> one(1, P);
+        if ((rc = dev_upload(A, one, &pl->dP))) return rc;
+    }
+    if (P.fft_smem > 65536) {
+        for (bool b : {false, true})
+            for (int st : {sht::kFFTSynthesis, sht::kFFTAnalysis})
+                (void)hipFuncSetAttribute(sht::fft_kernel(st, b), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          P.fft_smem);
+    }
     const size_t fb_bytes = (size_t)g.fb_rows * d.nlayers * 16;
     hipError_t e = hipMalloc(&pl->fbuf, fb_bytes);
     if (e != hipSuccess) return fail(SHT_ERR_ALLOC, std::string("hipMalloc scratch: ") + hipGetErrorString(e));
@@ -620,6 +694,17 @@ int sht_plan_destroy(sht_plan *plan) {
     return SHT_OK;
 }
 
+// Launches one stage kernel (every stage kernel has the same signature).
+static hipError_t launch_stage(const void *func, dim3 grid, unsigned smem, const DevPlan *plans, const CallArgs &c,
+                               const CallArgs *calls, const Tile *tiles, void *stream) {
+    const DevPlan *a0 = plans;
+    CallArgs a1 = c;
+    const CallArgs *a2 = calls;
+    const Tile *a3 = tiles;
+    void *args[] = {&a0, &a1, &a2, &a3};
+    return hipLaunchKernel(func, grid, dim3(256), args, smem, reinterpret_cast<hipStream_t>(stream));
+}
+
 static int check_call(sht_plan *plan, const void *a, const void *b, int K) {
     if (!plan) return fail(SHT_ERR_ARG, "null plan");
     if (!a || !b) return fail(SHT_ERR_ARG, "null data pointer");
@@ -636,17 +721,21 @@ int sht_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *fie
     if (ld_spec < plan->g.nharm) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_spec < nharm");
     if (ld_grid < plan->g.npoints) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_grid < npoints");
     DeviceGuard guard(plan->d.device);
+    const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
+    const CallArgs c{0, K, unscale_coslat ? 1 : 0, 0, coeffs, field, ld_spec, ld_grid, plan->fbuf};
+    hipError_t e;
     {
         ProfScope ps(plan, SHT_KERNEL_LEGENDRE_SYNTHESIS, stream);
-        rc = sht::launch_legendre_synthesis(plan->P, plan->d.precision, plan->syn_tiles, plan->nsyn, coeffs, ld_spec,
-                                            plan->fbuf, K, unscale_coslat ? 1 : 0, stream);
+        e = launch_stage(sht::legendre_kernel(sht::kLegendreSynthesis, plan->d.precision, false), dim3(plan->nsyn, cgs),
+                         0, plan->dP, c, nullptr, plan->syn_tiles, stream);
     }
-    if (rc) return fail(SHT_ERR_HIP, std::string("legendre synthesis launch: ") + hipGetErrorString((hipError_t)rc));
+    if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("legendre synthesis launch: ") + hipGetErrorString(e));
     {
         ProfScope ps(plan, SHT_KERNEL_FFT_SYNTHESIS, stream);
-        rc = sht::launch_fft_synthesis(plan->P, plan->fbuf, field, ld_grid, K, stream);
+        e = launch_stage(sht::fft_kernel(sht::kFFTSynthesis, false), dim3(plan->g.n, (K + plan->P.kg - 1) / plan->P.kg),
+                         plan->P.fft_smem, plan->dP, c, nullptr, nullptr, stream);
     }
-    if (rc) return fail(SHT_ERR_HIP, std::string("fft synthesis launch: ") + hipGetErrorString((hipError_t)rc));
+    if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("fft synthesis launch: ") + hipGetErrorString(e));
     return SHT_OK;
 }
 
@@ -657,17 +746,21 @@ int sht_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void *coeff
     if (ld_spec < plan->g.nharm) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_spec < nharm");
     if (ld_grid < plan->g.npoints) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_grid < npoints");
     DeviceGuard guard(plan->d.device);
+    const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
+    const CallArgs c{0, K, 0, 0, field, coeffs, ld_grid, ld_spec, plan->fbuf};
+    hipError_t e;
     {
         ProfScope ps(plan, SHT_KERNEL_FFT_ANALYSIS, stream);
-        rc = sht::launch_fft_analysis(plan->P, field, ld_grid, plan->fbuf, K, stream);
+        e = launch_stage(sht::fft_kernel(sht::kFFTAnalysis, false), dim3(plan->g.n, (K + plan->P.kg - 1) / plan->P.kg),
+                         plan->P.fft_smem, plan->dP, c, nullptr, nullptr, stream);
     }
-    if (rc) return fail(SHT_ERR_HIP, std::string("fft analysis launch: ") + hipGetErrorString((hipError_t)rc));
+    if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("fft analysis launch: ") + hipGetErrorString(e));
     {
         ProfScope ps(plan, SHT_KERNEL_LEGENDRE_ANALYSIS, stream);
-        rc = sht::launch_legendre_analysis(plan->P, plan->d.precision, plan->ana_tiles, plan->nana, plan->fbuf,
-                                           coeffs, ld_spec, K, stream);
+        e = launch_stage(sht::legendre_kernel(sht::kLegendreAnalysis, plan->d.precision, false), dim3(plan->nana, cgs),
+                         0, plan->dP, c, nullptr, plan->ana_tiles, stream);
     }
-    if (rc) return fail(SHT_ERR_HIP, std::string("legendre analysis launch: ") + hipGetErrorString((hipError_t)rc));
+    if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("legendre analysis launch: ") + hipGetErrorString(e));
     return SHT_OK;
 }
 
@@ -678,7 +771,7 @@ int sht_uv_from_vordiv(sht_plan *plan, const void *vor, const void *div, void *U
     if (!U || !V) return fail(SHT_ERR_ARG, "null data pointer");
     if (ld < plan->g.nharm) return fail(SHT_ERR_DIM, "DimensionMismatch: ld < nharm");
     DeviceGuard guard(plan->d.device);
-    rc = sht::launch_uv_from_vordiv(plan->P, vor, div, U, V, ld, K, (float)radius, stream);
+    rc = sht::launch_uv_from_vordiv(plan->dP, plan->g.nharm, vor, div, U, V, ld, K, (float)radius, stream);
     if (rc) return fail(SHT_ERR_HIP, std::string("UV_from_vordiv launch: ") + hipGetErrorString((hipError_t)rc));
     return SHT_OK;
 }
@@ -741,6 +834,222 @@ int sht_legendre_column(int32_t lmax, int32_t mmax, double x, double *out) {
         legendre_order(lmax, m, x, pmm, out + off);
         off += lmax - m;
     }
+    return SHT_OK;
+}
+
+}  // extern "C"
+
+// ================================================================== batches
+
+struct sht_batch {
+    int device = 0;
+    int mode = 0;
+    std::vector<void *> allocs;
+    const DevPlan *plans = nullptr;
+    const CallArgs *calls = nullptr;
+    const Tile *tiles[4] = {nullptr, nullptr, nullptr, nullptr};
+    int ntiles[4] = {0, 0, 0, 0};
+    unsigned fft_smem = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool use_graph = false;
+};
+
+namespace {
+
+void free_batch(sht_batch *b) {
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (void *a : b->allocs) (void)hipFree(a);
+    delete b;
+}
+
+const void *stage_func(const sht_batch *b, int st) {
+    return (st == sht::kLegendreSynthesis || st == sht::kLegendreAnalysis) ? sht::legendre_kernel(st, b->mode, true)
+                                                                            : sht::fft_kernel(st, true);
+}
+
+int build_batch(sht_batch *b, const sht_batch_item *items, int n, int flags) {
+    std::vector<sht_plan *> plans;
+    std::vector<CallArgs> calls;
+    std::vector<Tile> st_tiles[4];
+    std::vector<double> st_cost[4];
+    int rc;
+    // scratch: one Fourier buffer per item (a round trip's two calls share it)
+    std::vector<size_t> fb_off(n);
+    size_t fb_total = 0;
+    for (int i = 0; i < n; ++i) {
+        fb_off[i] = fb_total;
+        fb_total += (size_t)items[i].plan->g.fb_rows * items[i].K * 16;
+        fb_total = (fb_total + 255) & ~(size_t)255;
+    }
+    void *scratch = nullptr;
+    hipError_t e = hipMalloc(&scratch, std::max<size_t>(fb_total, 256));
+    if (e != hipSuccess) return fail(SHT_ERR_ALLOC, std::string("hipMalloc batch scratch: ") + hipGetErrorString(e));
+    b->allocs.push_back(scratch);
+    for (int i = 0; i < n; ++i) {
+        const sht_batch_item &it = items[i];
+        sht_plan *pl = it.plan;
+        int pi = (int)(std::find(plans.begin(), plans.end(), pl) - plans.begin());
+        if (pi == (int)plans.size()) plans.push_back(pl);
+        void *fb = static_cast<char *>(scratch) + fb_off[i];
+        const int K = it.K;
+        const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
+        const int kgs = (K + pl->P.kg - 1) / pl->P.kg;
+        auto add_fft = [&](int st, int call) {
+            for (int j : pl->h_ring_order)
+                for (int q = 0; q < kgs; ++q) {
+                    st_tiles[st].push_back({call, j, q, 0});
+                    st_cost[st].push_back(pl->h_ring_cost[j]);
+                }
+        };
+        auto add_leg = [&](int st, int call, const std::vector<Tile> &t, const std::vector<double> &c) {
+            for (size_t q = 0; q < t.size(); ++q)
+                for (int cg = 0; cg < cgs; ++cg) {
+                    st_tiles[st].push_back({call, t[q].m, t[q].blk, cg});
+                    st_cost[st].push_back(c[q]);
+                }
+        };
+        if (it.direction == SHT_DIR_SYNTHESIS || it.direction == SHT_DIR_ROUNDTRIP) {
+            void *grid = it.direction == SHT_DIR_ROUNDTRIP ? it.mid : it.out;
+            const long long ldg = it.direction == SHT_DIR_ROUNDTRIP ? it.ld_mid : it.ld_out;
+            calls.push_back({pi, K, it.unscale_coslat ? 1 : 0, 0, it.in, grid, it.ld_in, ldg, fb});
+            const int call = (int)calls.size() - 1;
+            add_leg(sht::kLegendreSynthesis, call, pl->h_syn, pl->h_syn_cost);
+            add_fft(sht::kFFTSynthesis, call);
+        }
+        if (it.direction == SHT_DIR_ANALYSIS || it.direction == SHT_DIR_ROUNDTRIP) {
+            const void *grid = it.direction == SHT_DIR_ROUNDTRIP ? it.mid : it.in;
+            const long long ldg = it.direction == SHT_DIR_ROUNDTRIP ? it.ld_mid : it.ld_in;
+            calls.push_back({pi, K, 0, 0, grid, it.out, ldg, it.ld_out, fb});
+            const int call = (int)calls.size() - 1;
+            add_fft(sht::kFFTAnalysis, call);
+            add_leg(sht::kLegendreAnalysis, call, pl->h_ana, pl->h_ana_cost);
+        }
+    }
+    std::vector<DevPlan> dplans;
+    for (sht_plan *pl : plans) {
+        dplans.push_back(pl->P);
+        b->fft_smem = std::max<unsigned>(b->fft_smem, (unsigned)pl->P.fft_smem);
+    }
+    if ((rc = dev_upload(b->allocs, dplans, &b->plans))) return rc;
+    if ((rc = dev_upload(b->allocs, calls, &b->calls))) return rc;
+    for (int st = 0; st < 4; ++st) {
+        // heaviest work first across all items
+        std::vector<size_t> idx(st_tiles[st].size());
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t c) { return st_cost[st][a] > st_cost[st][c]; });
+        std::vector<Tile> sorted(idx.size());
+        for (size_t q = 0; q < idx.size(); ++q) sorted[q] = st_tiles[st][idx[q]];
+        b->ntiles[st] = (int)sorted.size();
+        if ((rc = dev_upload(b->allocs, sorted, &b->tiles[st]))) return rc;
+    }
+    if (b->fft_smem > 65536) {
+        for (int st : {sht::kFFTSynthesis, sht::kFFTAnalysis})
+            (void)hipFuncSetAttribute(sht::fft_kernel(st, true), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)b->fft_smem);
+    }
+    if (flags & SHT_BATCH_NO_GRAPH) return SHT_OK;
+    // explicit graph: one kernel node per non-empty stage, chained in stage order
+    if (hipGraphCreate(&b->graph, 0) != hipSuccess) return SHT_OK;
+    hipGraphNode_t prev = nullptr;
+    CallArgs none{};
+    for (int st = 0; st < 4; ++st) {
+        if (b->ntiles[st] == 0) continue;
+        const DevPlan *a0 = b->plans;
+        CallArgs a1 = none;
+        const CallArgs *a2 = b->calls;
+        const Tile *a3 = b->tiles[st];
+        void *args[] = {&a0, &a1, &a2, &a3};
+        hipKernelNodeParams kp{};
+        kp.func = const_cast<void *>(stage_func(b, st));
+        kp.gridDim = dim3(b->ntiles[st]);
+        kp.blockDim = dim3(256);
+        kp.sharedMemBytes = (st == sht::kFFTSynthesis || st == sht::kFFTAnalysis) ? b->fft_smem : 0;
+        kp.kernelParams = args;
+        kp.extra = nullptr;
+        hipGraphNode_t node;
+        if (hipGraphAddKernelNode(&node, b->graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp) != hipSuccess) {
+            (void)hipGetLastError();
+            return SHT_OK;  // direct launches
+        }
+        prev = node;
+    }
+    if (flags & SHT_BATCH_FORCE_GRAPH_FAILURE) return SHT_OK;
+    if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        b->exec = nullptr;
+        return SHT_OK;
+    }
+    b->use_graph = true;
+    return SHT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sht_batch_create(const sht_batch_item *items, int32_t n, int32_t flags, sht_batch **batch) {
+    if (!batch) return fail(SHT_ERR_ARG, "null batch pointer");
+    *batch = nullptr;
+    if (!items || n < 1) return fail(SHT_ERR_ARG, "empty batch");
+    for (int i = 0; i < n; ++i) {
+        const sht_batch_item &it = items[i];
+        if (!it.plan) return fail(SHT_ERR_ARG, "batch item " + std::to_string(i) + ": null plan");
+        if (it.direction < SHT_DIR_SYNTHESIS || it.direction > SHT_DIR_ROUNDTRIP)
+            return fail(SHT_ERR_ARG, "batch item " + std::to_string(i) + ": bad direction");
+        if (it.plan->d.device != items[0].plan->d.device || it.plan->d.precision != items[0].plan->d.precision)
+            return fail(SHT_ERR_UNSUPPORTED, "batch plans must share device and precision");
+        const int rc = check_call(it.plan, it.in, it.out, it.K);
+        if (rc) return rc;
+        if (it.direction == SHT_DIR_ROUNDTRIP && !it.mid) return fail(SHT_ERR_ARG, "round trip needs a grid buffer");
+        const long long nh = it.plan->g.nharm, np = it.plan->g.npoints;
+        const bool spec_in = it.direction != SHT_DIR_ANALYSIS;
+        if (it.ld_in < (spec_in ? nh : np) || it.ld_out < (it.direction == SHT_DIR_SYNTHESIS ? np : nh) ||
+            (it.direction == SHT_DIR_ROUNDTRIP && it.ld_mid < np))
+            return fail(SHT_ERR_DIM, "DimensionMismatch: batch item " + std::to_string(i) + " column stride too small");
+    }
+    sht_batch *b = new (std::nothrow) sht_batch();
+    if (!b) return fail(SHT_ERR_ALLOC, "host allocation failed");
+    b->device = items[0].plan->d.device;
+    b->mode = items[0].plan->d.precision;
+    DeviceGuard guard(b->device);
+    const int rc = build_batch(b, items, n, flags);
+    if (rc) {
+        free_batch(b);
+        return rc;
+    }
+    *batch = b;
+    return SHT_OK;
+}
+
+int sht_batch_launch(sht_batch *b, void *stream) {
+    if (!b) return fail(SHT_ERR_ARG, "null batch");
+    DeviceGuard guard(b->device);
+    hipError_t e;
+    if (b->use_graph) {
+        e = hipGraphLaunch(b->exec, reinterpret_cast<hipStream_t>(stream));
+        if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+        return SHT_OK;
+    }
+    CallArgs none{};
+    for (int st = 0; st < 4; ++st) {
+        if (b->ntiles[st] == 0) continue;
+        const unsigned smem = (st == sht::kFFTSynthesis || st == sht::kFFTAnalysis) ? b->fft_smem : 0;
+        e = launch_stage(stage_func(b, st), dim3(b->ntiles[st]), smem, b->plans, none, b->calls, b->tiles[st], stream);
+        if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("batch stage launch: ") + hipGetErrorString(e));
+    }
+    return SHT_OK;
+}
+
+int sht_batch_uses_graph(const sht_batch *b) { return b && b->use_graph ? 1 : 0; }
+
+int sht_batch_destroy(sht_batch *b) {
+    if (!b) return SHT_OK;
+    DeviceGuard guard(b->device);
+    (void)hipDeviceSynchronize();
+    free_batch(b);
     return SHT_OK;
 }
 

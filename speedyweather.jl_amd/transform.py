@@ -176,3 +176,69 @@ def UV_from_vordiv_(U: LowerTriangularArray, V: LowerTriangularArray, vor: Lower
                                    ts[3][0].data_ptr(), ts[0][2], K, float(radius), S._stream()),
           "sht_uv_from_vordiv")
     return U, V
+
+
+class SpectralBatch:
+    """An uneven batch of independent transforms (any mix of SpectralTransforms, layer counts
+    and directions) run as one kernel launch per stage, replayed from a hipGraph.
+
+    items: sequences (S, inp, out) for a single direction -- (S, LowerTriangularArray, Field)
+    is spectral->grid, (S, Field, LowerTriangularArray) grid->spectral -- or (S, coeffs_in,
+    coeffs_out, mid_field) for a round trip; an optional trailing bool is unscale_coslat.
+    The tensors are bound at construction: every launch() repeats the same transforms."""
+
+    def __init__(self, items, graph: bool = True, force_graph_failure: bool = False):
+        arr = (_lib.ShtBatchItem * len(items))()
+        self._keep = []
+        self.items = list(items)
+        for i, it in enumerate(items):
+            S, inp, out = it[0], it[1], it[2]
+            rest = list(it[3:])
+            mid = rest.pop(0) if rest and isinstance(rest[0], Field) else None
+            unscale = bool(rest[0]) if rest else False
+            if mid is not None:
+                direction = _lib.SHT_DIR_ROUNDTRIP
+                a, K, ld_in = S._check_tensor(inp.data, S.spectrum.nharm, torch.complex64, "coeffs")
+                m2, _, ld_mid = S._check_tensor(mid.data, S.grid.npoints, torch.float32, "field")
+                o, _, ld_out = S._check_tensor(out.data, S.spectrum.nharm, torch.complex64, "coeffs")
+                checks = (inp, mid, out)
+            elif isinstance(inp, LowerTriangularArray):
+                direction = _lib.SHT_DIR_SYNTHESIS
+                a, K, ld_in = S._check_tensor(inp.data, S.spectrum.nharm, torch.complex64, "coeffs")
+                o, _, ld_out = S._check_tensor(out.data, S.grid.npoints, torch.float32, "field")
+                m2, ld_mid, checks = None, 0, (inp, out)
+            else:
+                direction = _lib.SHT_DIR_ANALYSIS
+                a, K, ld_in = S._check_tensor(inp.data, S.grid.npoints, torch.float32, "field")
+                o, _, ld_out = S._check_tensor(out.data, S.spectrum.nharm, torch.complex64, "coeffs")
+                m2, ld_mid, checks = None, 0, (inp, out)
+            for x in checks:
+                if not S.ismatching(x):
+                    raise _mismatch(S, x)
+            arr[i] = _lib.ShtBatchItem(S._plan.value, direction, K, int(unscale), 0, a.data_ptr(), ld_in,
+                                       m2.data_ptr() if m2 is not None else None, ld_mid, o.data_ptr(), ld_out)
+            self._keep.append((S, a, m2, o))
+        flags = (0 if graph else _lib.SHT_BATCH_NO_GRAPH) | (
+            _lib.SHT_BATCH_FORCE_GRAPH_FAILURE if force_graph_failure else 0)
+        self._batch = C.c_void_p()
+        self.device = items[0][0].device
+        check(lib().sht_batch_create(arr, len(items), flags, C.byref(self._batch)), "sht_batch_create")
+
+    @property
+    def uses_graph(self) -> bool:
+        return bool(lib().sht_batch_uses_graph(self._batch))
+
+    def launch(self):
+        check(lib().sht_batch_launch(self._batch, C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)),
+              "sht_batch_launch")
+
+    def close(self):
+        if getattr(self, "_batch", None) and self._batch.value:
+            lib().sht_batch_destroy(self._batch)
+            self._batch = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
