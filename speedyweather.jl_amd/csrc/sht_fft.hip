@@ -374,7 +374,7 @@ __device__ void fft_pass_mfma_first(float2 *buf, const float2 *__restrict__ csta
 // Runs all passes in place on buf (ncol columns of N points).  A first-pass MFMA radix
 // runs across the workgroup; every other pass runs per column, column c on wave c % 4.
 template <bool INV>
-__device__ void fft_run(float2 *buf, const float2 *tw, const float2 *cstab, const int *fac, int nf, int N, int ncol,
+__device__ __attribute__((always_inline)) void fft_run(float2 *buf, const float2 *tw, const float2 *cstab, const int *fac, int nf, int N, int ncol,
                         int tid) {
     int l = 1, f = 0;
     if (nf > 0 && fac[0] >= kMfmaMinRadix) {
@@ -428,7 +428,7 @@ __device__ inline void load_twiddles(const DevPlan &P, int j, int N, float2 *tw,
 // one complex inverse FFT of Z = g_N + i g_S (Hermitian-extended; Im of the DC and
 // Nyquist bins ignored as FFTW's c2r does; orders above mt are zero).
 template <bool BATCH>
-__global__ __launch_bounds__(kFFTThreads, 5) void fft_synthesis_kernel(const DevPlan *__restrict__ plans,
+__global__ __launch_bounds__(kFFTThreads, 4) void fft_synthesis_kernel(const DevPlan *__restrict__ plans,
                                                                     CallArgs single,
                                                                     const CallArgs *__restrict__ calls,
                                                                     const Tile *__restrict__ tiles) {
@@ -501,7 +501,7 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_synthesis_kernel(const Dev
 // southern ring as one complex FFT of z = f_N + i f_S, separated by symmetry into f_N, f_S
 // and stored as (f_N + f_S, f_N - f_S) for orders 0..mt.
 template <bool BATCH>
-__global__ __launch_bounds__(kFFTThreads, 5) void fft_analysis_kernel(const DevPlan *__restrict__ plans,
+__global__ __launch_bounds__(kFFTThreads, 4) void fft_analysis_kernel(const DevPlan *__restrict__ plans,
                                                                    CallArgs single,
                                                                    const CallArgs *__restrict__ calls,
                                                                    const Tile *__restrict__ tiles) {

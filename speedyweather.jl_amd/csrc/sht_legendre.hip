@@ -127,7 +127,7 @@ using V8of = typename Split<MODE == kFP32 ? kFP16x3 : MODE>::V;
 // consecutive degrees of one complex column (kk = tid/8, degrees 8(tid%8)..+7).
 
 template <int MODE, bool BATCH>
-__global__ __launch_bounds__(256, 4) void legendre_synthesis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
+__global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
                                                                  const CallArgs *__restrict__ calls,
                                                                  const Tile *__restrict__ tiles) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(256, 4) void legendre_synthesis_kernel(const DevPla
 // is written exactly once (no atomics, deterministic).
 
 template <int MODE, bool BATCH>
-__global__ __launch_bounds__(256, 4) void legendre_analysis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
+__global__ __launch_bounds__(256) void legendre_analysis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
                                                                 const CallArgs *__restrict__ calls,
                                                                 const Tile *__restrict__ tiles) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
