@@ -263,107 +263,161 @@ __device__ void fft_pass_wave(float2 *col, const float2 *tw, int N, int l, int l
 // b_t = x_t - x_{r-t} (t = 1..h, h = (r-1)/2):
 //   T1[s] = sum_t cos(2 pi s t / r) a_t   (s = 0..h),   T2[s] = sum_t sin(2 pi s t / r) b_t
 //   y_0 = x_0 + T1[0],  y_s = x_0 + T1[s] -+ i T2[s],  y_{r-s} = x_0 + T1[s] +- i T2[s]
-// (upper signs forward), y_s -> buf[b + M s].  GEMM rows = s (M-tiles of 16), columns =
-// (butterfly, re/im) (N-tiles of 16), K = t (steps of 4); tiles go round-robin to the
-// waves.  The cos/sin
-// operand is read from a per-prime table in MFMA-fragment order (one float2 per lane and
-// K-step, loaded one step ahead).  Butterflies read and write only their own r slots, so
-// the N-tiles are processed in rounds: all tiles of a round are computed before any of
-// their outputs is written.
-template <bool INV>
-__device__ void fft_pass_mfma_first(float2 *buf, const float2 *__restrict__ cstab, int N, int r, int ncol,
-                                    int tid) {
-    const int M = N / r;  // butterflies per column
-    const int h = (r - 1) >> 1;
-    const int nseq = ncol * M;
-    const FastDiv divH(h), divM(M);
-    for (int it = tid; it < nseq * h; it += kFFTThreads) {
-        const int sq = divH.div(it), t = it - sq * h + 1;
-        const int col = divM.div(sq), b = sq - col * M;
-        float2 *x = buf + col * N + b;
-        const float2 xt = x[M * t], xr = x[M * (r - t)];
-        x[M * t] = cadd(xt, xr);
-        x[M * (r - t)] = csub(xt, xr);
-    }
-    __syncthreads();
-    const int lane = tid & 63, wave = tid >> 6, row = lane & 15, grp = lane >> 4;
-    constexpr int NW = kFFTThreads / 64;
-    constexpr int TPW = kMfmaTilesPerWave;
-    const int Mt = (h + 16) >> 4;         // rows 0..h
-    const int Nt = (2 * nseq + 15) >> 4;  // real columns
+// (upper signs forward), y_s -> buf[b + M s].  GEMM rows = s (row tiles of 16), columns =
+// (butterfly, re/im) (column tiles of 16), K = t (steps of 4).
+//
+// Each wave owns one or two row tiles (or, with fewer than four row tiles, a share of the
+// column tiles of one row tile) and sweeps its column tiles in rounds of MT x NT tiles, so
+// one cos/sin operand serves NT column tiles per K-step.  The cos/sin operand of lane
+// (row s, t) is the ring's twiddle w_N^{(s t mod r) M} from LDS, its index advanced by
+// 4 s mod r per K-step; all operands are loaded two K-steps ahead of their MFMAs.
+// Butterflies read and write only their own r slots, so all tiles of a round are computed
+// before any of their outputs is written.
+template <int MT, int NT>
+struct MfmaRound {
+    f4 t1[MT][NT], t2[MT][NT];
+};
+
+template <bool INV, int MT, int NT>
+__device__ inline void mfma_prime_rounds(float2 *buf, const float2 *tw, int N, int r, int M, int h, int nseq,
+                                         int mt0, int mstep, int nmt, int nt0, int ntstep, int ntround, int Nt,
+                                         int lane) {
+    const int row = lane & 15, grp = lane >> 4;
     const int Kt = (h + 3) >> 2;
-    const int NR = max(1, (TPW * NW) / Mt);  // column tiles per round
-    for (int nt0 = 0; nt0 < Nt; nt0 += NR) {
-        const int ntiles = Mt * min(NR, Nt - nt0);
-        const int nq = ntiles > wave ? (ntiles - wave + NW - 1) / NW : 0;  // tiles of this wave
-        f4 t1[TPW], t2[TPW];
-        int xo[TPW], ao[TPW];  // float offset of the column's input in buf, cos/sin row offset
-        bool colok[TPW];
-        float2 x0[TPW], anext[TPW];
+    const int t0 = grp + 1;
+    const FastDiv divM(M);
+    const float *bf = reinterpret_cast<const float *>(buf);
+    // cos/sin operand state per owned row tile
+    int e0[MT], de[MT];
 #pragma unroll
-        for (int q = 0; q < TPW; ++q) {
-            const int tile = wave + q * NW;
-            const int mt = tile % Mt, nt = nt0 + tile / Mt;
-            const int c = nt * 16 + row;
-            const int sq = c >> 1;
-            colok[q] = q < nq && sq < nseq;
-            int base = 0;
-            x0[q] = make_float2(0.f, 0.f);
-            if (colok[q]) {
+    for (int m = 0; m < MT; ++m) {
+        const int s = min((mt0 + m * mstep) * 16 + row, r - 1);  // rows above h are discarded
+        e0[m] = ((s * t0) % r) * M;
+        de[m] = ((4 * s) % r) * M;
+    }
+    const bool last_ok = t0 + 4 * (Kt - 1) <= h;  // this lane's t in the last K-step is used
+    for (int rb = 0; rb * ntround < Nt; ++rb) {
+        int pa[NT], pb[NT];  // float offsets of this lane's a_t, b_t in buf
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            const int nt = nt0 + rb * ntround + n * ntstep;
+            const int c = nt * 16 + row, sq = c >> 1;
+            int base = 0;  // columns past the data read column 0; their outputs are dropped
+            if (sq < nseq) {
                 const int col = divM.div(sq);
                 base = col * N + (sq - col * M);
-                x0[q] = buf[base];
             }
-            xo[q] = 2 * base + (c & 1);
-            t1[q] = f4{0.f, 0.f, 0.f, 0.f};
-            t2[q] = f4{0.f, 0.f, 0.f, 0.f};
-            ao[q] = mt * Kt * 64 + lane;
-            anext[q] = q < nq ? cstab[ao[q]] : make_float2(0.f, 0.f);
+            pa[n] = 2 * base + (c & 1) + 2 * M * t0;
+            pb[n] = 2 * base + (c & 1) + 2 * M * (r - t0);
         }
-        const float *bf = reinterpret_cast<const float *>(buf);
-        for (int ks = 0; ks < Kt; ++ks) {
-            const int t = ks * 4 + grp + 1;
-            const bool tok = t <= h;
+        int e[MT];
+        MfmaRound<MT, NT> acc;
 #pragma unroll
-            for (int q = 0; q < TPW; ++q) {
-                if (q >= nq) break;  // uniform per wave
-                const float2 a = anext[q];
-                if (ks + 1 < Kt) anext[q] = cstab[ao[q] + (ks + 1) * 64];
-                const float av = (tok && colok[q]) ? bf[xo[q] + 2 * M * t] : 0.f;
-                const float bv = (tok && colok[q]) ? bf[xo[q] + 2 * M * (r - t)] : 0.f;
-                t1[q] = mfma_f32(a.x, av, t1[q]);
-                t2[q] = mfma_f32(a.y, bv, t2[q]);
+        for (int m = 0; m < MT; ++m) {
+            e[m] = e0[m];
+#pragma unroll
+            for (int n = 0; n < NT; ++n) {
+                acc.t1[m][n] = f4{0.f, 0.f, 0.f, 0.f};
+                acc.t2[m][n] = f4{0.f, 0.f, 0.f, 0.f};
+            }
+        }
+        // operands of K-steps ks (slot 0) and ks + 1 (slot 1)
+        float2 A0[MT], A1[MT];
+        float X0[NT], Y0[NT], X1[NT], Y1[NT];
+        auto fetch = [&](float2 *A, float *X, float *Y) {
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+                A[m] = tw[e[m]];
+                e[m] += de[m];
+                e[m] -= e[m] >= N ? N : 0;
+            }
+#pragma unroll
+            for (int n = 0; n < NT; ++n) {
+                X[n] = bf[pa[n]];
+                Y[n] = bf[pb[n]];
+                pa[n] += 8 * M;
+                pb[n] -= 8 * M;
+            }
+        };
+        fetch(A0, X0, Y0);
+        if (Kt > 1) fetch(A1, X1, Y1);
+        for (int ks = 0; ks < Kt; ++ks) {
+            float2 A2[MT];
+            float X2[NT], Y2[NT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m) A2[m] = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int n = 0; n < NT; ++n) X2[n] = Y2[n] = 0.f;
+            if (ks + 2 < Kt) fetch(A2, X2, Y2);
+            if (ks == Kt - 1 && !last_ok) {
+#pragma unroll
+                for (int m = 0; m < MT; ++m) A0[m] = make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+                if (m >= nmt) break;  // uniform per wave
+                // the table holds w^k = cos - i sin (forward twiddles)
+                const float cs = A0[m].x, sn = -A0[m].y;
+#pragma unroll
+                for (int n = 0; n < NT; ++n) {
+                    acc.t1[m][n] = mfma_f32(cs, X0[n], acc.t1[m][n]);
+                    acc.t2[m][n] = mfma_f32(sn, Y0[n], acc.t2[m][n]);
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+                A0[m] = A1[m];
+                A1[m] = A2[m];
+            }
+#pragma unroll
+            for (int n = 0; n < NT; ++n) {
+                X0[n] = X1[n];
+                Y0[n] = Y1[n];
+                X1[n] = X2[n];
+                Y1[n] = Y2[n];
+            }
+        }
+        // x_0 of each column's butterfly, read before any output of the round is written
+        int ob[NT];
+        float2 z0[NT];
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            const int nt = nt0 + rb * ntround + n * ntstep;
+            const int sq = (nt * 16 + row) >> 1;
+            ob[n] = -1;
+            z0[n] = make_float2(0.f, 0.f);
+            if (nt < Nt && sq < nseq) {
+                const int col = divM.div(sq);
+                ob[n] = col * N + (sq - col * M);
+                z0[n] = buf[ob[n]];
             }
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < TPW; ++q) {
-            if (q >= nq) break;  // uniform per wave
-            const int tile = wave + q * NW;
-            const int mt = tile % Mt, nt = nt0 + tile / Mt;
-            const int c = nt * 16 + row;
-            const int sq = c >> 1, reim = c & 1;
-            int ob = 0;
-            if (colok[q]) {
-                const int col = divM.div(sq);
-                ob = col * N + (sq - col * M);
-            }
+        for (int m = 0; m < MT; ++m) {
+            if (m >= nmt) break;  // uniform per wave
+            const int mt = mt0 + m * mstep;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const float a1 = t1[q][i], a2 = t2[q][i];
-                const float p1 = __shfl_xor(a1, 1), p2 = __shfl_xor(a2, 1);
-                const int s = mt * 16 + 4 * grp + i;
-                if (!colok[q] || s > h) continue;
-                const float T1re = reim ? p1 : a1, T1im = reim ? a1 : p1;
-                const float T2re = reim ? p2 : a2, T2im = reim ? a2 : p2;
-                const float sg = INV ? -1.f : 1.f;
-                const float2 z = x0[q];
-                if (s == 0) {
-                    if (reim == 0) buf[ob] = make_float2(z.x + T1re, z.y + T1im);
-                } else if (reim == 0) {
-                    buf[ob + M * s] = make_float2(z.x + T1re + sg * T2im, z.y + T1im - sg * T2re);
-                } else {
-                    buf[ob + M * (r - s)] = make_float2(z.x + T1re - sg * T2im, z.y + T1im + sg * T2re);
+            for (int n = 0; n < NT; ++n) {
+                const int reim = row & 1;
+                if (ob[n] < 0) continue;
+                const float2 z = z0[n];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float a1 = acc.t1[m][n][i], a2 = acc.t2[m][n][i];
+                    const float p1 = __shfl_xor(a1, 1), p2 = __shfl_xor(a2, 1);
+                    const int s = mt * 16 + 4 * grp + i;
+                    if (s > h) continue;
+                    const float T1re = reim ? p1 : a1, T1im = reim ? a1 : p1;
+                    const float T2re = reim ? p2 : a2, T2im = reim ? a2 : p2;
+                    const float sg = INV ? -1.f : 1.f;
+                    if (s == 0) {
+                        if (reim == 0) buf[ob[n]] = make_float2(z.x + T1re, z.y + T1im);
+                    } else if (reim == 0) {
+                        buf[ob[n] + M * s] = make_float2(z.x + T1re + sg * T2im, z.y + T1im - sg * T2re);
+                    } else {
+                        buf[ob[n] + M * (r - s)] = make_float2(z.x + T1re - sg * T2im, z.y + T1im + sg * T2re);
+                    }
                 }
             }
         }
@@ -371,14 +425,56 @@ __device__ void fft_pass_mfma_first(float2 *buf, const float2 *__restrict__ csta
     }
 }
 
+template <bool INV>
+__device__ void fft_pass_mfma_first(float2 *buf, const float2 *tw, int N, int r, int ncol, int tid) {
+    const int M = N / r;  // butterflies per column
+    const int h = (r - 1) >> 1;
+    const int nseq = ncol * M;
+    {
+        const FastDiv divH(h), divM(M);
+        for (int it = tid; it < nseq * h; it += kFFTThreads) {
+            const int sq = divH.div(it), t = it - sq * h + 1;
+            const int col = divM.div(sq), b = sq - col * M;
+            float2 *x = buf + col * N + b;
+            const float2 xt = x[M * t], xr = x[M * (r - t)];
+            x[M * t] = cadd(xt, xr);
+            x[M * (r - t)] = csub(xt, xr);
+        }
+    }
+    __syncthreads();
+    constexpr int NW = kFFTThreads / 64;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int Mt = (h + 16) >> 4;         // row tiles (rows 0..h), at most 2 NW
+    const int Nt = (2 * nseq + 15) >> 4;  // column tiles over (butterfly, re/im)
+    if (Mt > NW) {
+        // waves own row tiles wave and wave + NW; rounds of 2 column tiles
+        const int nmt = wave + NW < Mt ? 2 : 1;
+        mfma_prime_rounds<INV, 2, 2>(buf, tw, N, r, M, h, nseq, wave, NW, nmt, 0, 1, 2, Nt, lane);
+    } else {
+        // G = NW / Mt waves share a row tile, each taking every G-th column tile
+        const int G = NW / Mt;
+        const int grp = wave / Mt;
+        if (grp < G) {
+            mfma_prime_rounds<INV, 1, 4>(buf, tw, N, r, M, h, nseq, wave % Mt, 0, 1, grp, G, 4 * G, Nt, lane);
+        } else {
+            // idle wave: still takes part in every round's two barriers
+            const int rounds = (Nt + 4 * G - 1) / (4 * G);
+            for (int rb = 0; rb < rounds; ++rb) {
+                __syncthreads();
+                __syncthreads();
+            }
+        }
+    }
+}
+
 // Runs all passes in place on buf (ncol columns of N points).  A first-pass MFMA radix
 // runs across the workgroup; every other pass runs per column, column c on wave c % 4.
 template <bool INV>
-__device__ __attribute__((always_inline)) void fft_run(float2 *buf, const float2 *tw, const float2 *cstab, const int *fac, int nf, int N, int ncol,
+__device__ __attribute__((always_inline)) void fft_run(float2 *buf, const float2 *tw, const int *fac, int nf, int N, int ncol,
                         int tid) {
     int l = 1, f = 0;
     if (nf > 0 && fac[0] >= kMfmaMinRadix) {
-        fft_pass_mfma_first<INV>(buf, cstab, N, fac[0], ncol, tid);
+        fft_pass_mfma_first<INV>(buf, tw, N, fac[0], ncol, tid);
         l = fac[0];
         f = 1;
     }
@@ -483,7 +579,7 @@ __global__ __launch_bounds__(kFFTThreads, 4) void fft_synthesis_kernel(const Dev
         }
     }
     __syncthreads();
-    fft_run<true>(b0, tw, P.cstab + P.cs_off[j], P.fac + j * kMaxFactors, P.nfac[j], N, ncol, tid);
+    fft_run<true>(b0, tw, P.fac + j * kMaxFactors, P.nfac[j], N, ncol, tid);
     const float2 *res = b0;
     const long long rn = P.ring_n[j], rs = P.ring_s[j];
     const int lane = tid & 63;
@@ -543,7 +639,7 @@ __global__ __launch_bounds__(kFFTThreads, 4) void fft_analysis_kernel(const DevP
         }
     }
     __syncthreads();
-    fft_run<false>(b0, tw, P.cstab + P.cs_off[j], P.fac + j * kMaxFactors, P.nfac[j], N, ncol, tid);
+    fft_run<false>(b0, tw, P.fac + j * kMaxFactors, P.nfac[j], N, ncol, tid);
     __syncthreads();
     const float2 *res = b0;
     const long long fb0 = P.fboff[j];

@@ -100,11 +100,6 @@ struct DevPlan {
     const long long *tw_off;
     const int *fac;        // kMaxFactors radices per ring
     const int *nfac;
-    // cos/sin operand of the MFMA radix pass, per prime in fragment order
-    // [row tile][K-step][lane] = (cos, sin)(2 pi s t / r), s = 16 tile + lane%16,
-    // t = 4 step + lane/16 + 1 (zero outside s, t <= (r-1)/2); ring j's table at cs_off[j]
-    const float2 *cstab;
-    const long long *cs_off;
 
     // UV_from_vordiv factors (per packed lm): x = -m/(l(l+1)), u1 = eps_lm / l,
     // u2 = eps_{l+1,m} / (l+1) (0 on the last row)

@@ -469,34 +469,6 @@ int build_plan(sht_plan *pl) {
             tw[2 * (tw_off[j] + k) + 1] = (float)std::sin(a);
         }
     }
-    // cos/sin fragment tables of the first-pass MFMA primes (shared by rings with one prime)
-    std::vector<float2> cstab;
-    std::vector<long long> cs_off(n, 0);
-    {
-        std::vector<long long> prime_off(sht::kMaxMfmaRadix + 1, -1);
-        for (int j = 0; j < n; ++j) {
-            const int r = nfac[j] > 0 ? fac[(size_t)j * sht::kMaxFactors] : 1;
-            if (r < sht::kMfmaMinRadix) continue;
-            if (prime_off[r] < 0) {
-                prime_off[r] = (long long)cstab.size();
-                const int h = (r - 1) / 2, Mt = (h + 16) / 16, Kt = (h + 3) / 4;
-                for (int mt = 0; mt < Mt; ++mt)
-                    for (int ks = 0; ks < Kt; ++ks)
-                        for (int lane = 0; lane < 64; ++lane) {
-                            const int s = 16 * mt + (lane & 15), t = 4 * ks + (lane >> 4) + 1;
-                            float2 v = {0.f, 0.f};
-                            if (s <= h && t <= h) {
-                                const double a = 2.0 * M_PI * (double)((long long)s * t % r) / r;
-                                v.x = (float)std::cos(a);
-                                v.y = (float)std::sin(a);
-                            }
-                            cstab.push_back(v);
-                        }
-            }
-            cs_off[j] = prime_off[r];
-        }
-        if (cstab.empty()) cstab.push_back({0.f, 0.f});
-    }
     std::vector<double> ring_cost(n);
     for (int j = 0; j < n; ++j) {
         double c = 0;
@@ -599,8 +571,6 @@ int build_plan(sht_plan *pl) {
     if ((rc = dev_upload(A, tw_off, &P.tw_off))) return rc;
     if ((rc = dev_upload(A, fac, &P.fac))) return rc;
     if ((rc = dev_upload(A, nfac, &P.nfac))) return rc;
-    if ((rc = dev_upload(A, cstab, &P.cstab))) return rc;
-    if ((rc = dev_upload(A, cs_off, &P.cs_off))) return rc;
     if ((rc = dev_upload(A, uvx, &P.uv_x))) return rc;
     if ((rc = dev_upload(A, uv1, &P.uv_1))) return rc;
     if ((rc = dev_upload(A, uv2, &P.uv_2))) return rc;
