@@ -13,6 +13,38 @@ typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16;
 typedef u16 u16x8 __attribute__((ext_vector_type(8)));
 
+// Global-memory view of a generic pointer.  Buffers reached through structures (plan
+// tables, call arguments) are generic pointers to the compiler, which then emits FLAT
+// accesses that count against both the memory and the LDS counters and serialise the
+// loads around every LDS access; the cast lets it emit GLOBAL accesses instead.
+#define SHT_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const SHT_GLOBAL T *as_global(const T *p) {
+    return (const SHT_GLOBAL T *)p;
+}
+template <class T>
+__device__ __forceinline__ SHT_GLOBAL T *as_global(T *p) {
+    return (SHT_GLOBAL T *)p;
+}
+// float2 / float4 are classes whose members take generic `this`; global accesses of them go
+// through the equivalent builtin vectors.
+typedef float fv2 __attribute__((ext_vector_type(2)));
+typedef float fv4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float2 gload(const SHT_GLOBAL float2 *p) {
+    const fv2 v = *reinterpret_cast<const SHT_GLOBAL fv2 *>(p);
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float4 gload(const SHT_GLOBAL float4 *p) {
+    const fv4 v = *reinterpret_cast<const SHT_GLOBAL fv4 *>(p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gstore(SHT_GLOBAL float2 *p, float2 v) {
+    *reinterpret_cast<SHT_GLOBAL fv2 *>(p) = fv2{v.x, v.y};
+}
+__device__ __forceinline__ void gstore(SHT_GLOBAL float4 *p, float4 v) {
+    *reinterpret_cast<SHT_GLOBAL fv4 *>(p) = fv4{v.x, v.y, v.z, v.w};
+}
+
 // ------------------------------------------------------------------ MFMA helpers
 
 __device__ inline f4 mfma_f32(float a, float b, f4 c) {

@@ -470,7 +470,7 @@ __device__ void fft_pass_mfma_first(float2 *buf, const float2 *tw, int N, int r,
 // Runs all passes in place on buf (ncol columns of N points).  A first-pass MFMA radix
 // runs across the workgroup; every other pass runs per column, column c on wave c % 4.
 template <bool INV>
-__device__ __attribute__((always_inline)) void fft_run(float2 *buf, const float2 *tw, const int *fac, int nf, int N, int ncol,
+__device__ __attribute__((always_inline)) void fft_run(float2 *buf, const float2 *tw, const SHT_GLOBAL int *fac, int nf, int N, int ncol,
                         int tid) {
     int l = 1, f = 0;
     if (nf > 0 && fac[0] >= kMfmaMinRadix) {
@@ -503,28 +503,46 @@ __device__ __attribute__((always_inline)) void fft_run(float2 *buf, const float2
 
 // Twiddles of ring j into LDS with all loads of a thread in flight together.
 __device__ inline void load_twiddles(const DevPlan &P, int j, int N, float2 *tw, int tid) {
-    const float2 *gtw = reinterpret_cast<const float2 *>(P.twiddle) + P.tw_off[j];
-    constexpr int U = 4;
-    for (int i0 = 0; i0 < N; i0 += U * kFFTThreads) {
-        float2 v[U];
+    const SHT_GLOBAL float2 *gtw = as_global(reinterpret_cast<const float2 *>(P.twiddle)) + as_global(P.tw_off)[j];
+    constexpr int U = (kFFTMaxRing + kFFTThreads - 1) / kFFTThreads;
+    float2 v[U];
 #pragma unroll
-        for (int q = 0; q < U; ++q) {
-            const int i = i0 + q * kFFTThreads + tid;
-            if (i < N) v[q] = gtw[i];
-        }
+    for (int q = 0; q < U; ++q) {
+        const int i = q * kFFTThreads + tid;
+        if (i < N) v[q] = gload(gtw + i);
+    }
 #pragma unroll
-        for (int q = 0; q < U; ++q) {
-            const int i = i0 + q * kFFTThreads + tid;
-            if (i < N) tw[i] = v[q];
-        }
+    for (int q = 0; q < U; ++q) {
+        const int i = q * kFFTThreads + tid;
+        if (i < N) tw[i] = v[q];
     }
 }
+
+// Per-ring launch geometry of the FFT kernels: ring j, first column k0, columns ncol.
+struct FFTTile {
+    int j, N, mt, k0, ncol;
+};
+
+template <bool BATCH>
+__device__ inline FFTTile fft_tile(const DevPlan &P, const Tile &T, int K) {
+    FFTTile t;
+    t.j = BATCH ? T.m : as_global(P.ring_order)[blockIdx.y];
+    t.N = as_global(P.nlon)[t.j];
+    t.mt = as_global(P.mt)[t.j];
+    t.k0 = (BATCH ? T.blk : (int)blockIdx.x) * kFFTCols;
+    t.ncol = min(kFFTCols, K - t.k0);
+    return t;
+}
+
+// Fourier rows are read and written with thread -> (order mm = tid/4 + 64 u, column tid%4).
+constexpr int kFFTRowsPerPass = kFFTThreads / kFFTCols;
+constexpr int kFFTRowPasses = (kFFTMaxRing / 2 + kFFTRowsPerPass) / kFFTRowsPerPass;
 
 // spectral -> grid, Fourier stage (fourier.jl:215-250): per ring, brfft of g_N and g_S as
 // one complex inverse FFT of Z = g_N + i g_S (Hermitian-extended; Im of the DC and
 // Nyquist bins ignored as FFTW's c2r does; orders above mt are zero).
 template <bool BATCH>
-__global__ __launch_bounds__(kFFTThreads, 4) void fft_synthesis_kernel(const DevPlan *__restrict__ plans,
+__global__ __launch_bounds__(kFFTThreads, 5) void fft_synthesis_kernel(const DevPlan *__restrict__ plans,
                                                                     CallArgs single,
                                                                     const CallArgs *__restrict__ calls,
                                                                     const Tile *__restrict__ tiles) {
@@ -533,62 +551,63 @@ __global__ __launch_bounds__(kFFTThreads, 4) void fft_synthesis_kernel(const Dev
     const Tile T = BATCH ? tiles[blockIdx.x] : Tile{0, 0, 0, 0};
     const CallArgs C = BATCH ? calls[T.call] : single;
     const DevPlan &P = plans[C.plan];
-    const float4 *__restrict__ fbuf = static_cast<const float4 *>(C.fbuf);
-    float *__restrict__ field = static_cast<float *>(C.out);
-    const long long ld_grid = C.ld_out;
     const int K = C.K;
-    const int j = BATCH ? T.m : P.ring_order[blockIdx.x];
-    const int N = P.nlon[j], mt = P.mt[j];
-    const int k0 = (BATCH ? T.blk : (int)blockIdx.y) * P.kg;
-    const int ncol = min(P.kg, K - k0);
+    const FFTTile G = fft_tile<BATCH>(P, T, K);
+    const int N = G.N, mt = G.mt, ncol = G.ncol;
     float2 *tw = smem;
     float2 *b0 = smem + N;
-    load_twiddles(P, j, N, tw, tid);
-    const long long fb0 = P.fboff[j];
-    const int half = N / 2;
-    const int items = ncol * (half + 1);
-    const FastDiv divC(ncol);
-    constexpr int U = 8;
-    for (int i0 = 0; i0 < items; i0 += U * kFFTThreads) {
-        float4 v[U];
+    load_twiddles(P, G.j, N, tw, tid);
+    {
+        const SHT_GLOBAL float4 *fb =
+            as_global(static_cast<const float4 *>(C.fbuf)) + as_global(P.fboff)[G.j] * (long long)K + G.k0;
+        const int half = N / 2;
+        const int col = tid % kFFTCols, m0 = tid / kFFTCols;
+        float4 v[kFFTRowPasses];
 #pragma unroll
-        for (int q = 0; q < U; ++q) {
-            const int it = i0 + q * kFFTThreads + tid;
-            v[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (it < items) {
-                const int mm = divC.div(it), col = it - mm * ncol;
-                if (mm <= mt) v[q] = fbuf[(fb0 + mm) * (long long)K + k0 + col];
-            }
+        for (int u = 0; u < kFFTRowPasses; ++u) {
+            const int mm = m0 + kFFTRowsPerPass * u;
+            v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (col < ncol && mm <= mt) v[u] = gload(fb + (unsigned)(mm * K + col));
         }
+        float2 *z = b0 + col * N;
 #pragma unroll
-        for (int q = 0; q < U; ++q) {
-            const int it = i0 + q * kFFTThreads + tid;
-            if (it < items) {
-                const int mm = divC.div(it), col = it - mm * ncol;
-                float2 *z = b0 + col * N;
-                const float4 w = v[q];
-                if (mm == 0) {
-                    z[0] = make_float2(w.x, w.z);
-                } else if (mm < half) {
-                    z[mm] = make_float2(w.x - w.w, w.y + w.z);
-                    z[N - mm] = make_float2(w.x + w.w, w.z - w.y);
-                } else {
-                    z[half] = make_float2(w.x, w.z);
-                }
+        for (int u = 0; u < kFFTRowPasses; ++u) {
+            const int mm = m0 + kFFTRowsPerPass * u;
+            if (col >= ncol || mm > half) continue;
+            const float4 w = v[u];
+            if (mm == 0) {
+                z[0] = make_float2(w.x, w.z);
+            } else if (mm < half) {
+                z[mm] = make_float2(w.x - w.w, w.y + w.z);
+                z[N - mm] = make_float2(w.x + w.w, w.z - w.y);
+            } else {
+                z[half] = make_float2(w.x, w.z);
             }
         }
     }
     __syncthreads();
-    fft_run<true>(b0, tw, P.fac + j * kMaxFactors, P.nfac[j], N, ncol, tid);
-    const float2 *res = b0;
-    const long long rn = P.ring_n[j], rs = P.ring_s[j];
-    const int lane = tid & 63;
-    for (int col = tid >> 6; col < ncol; col += kFFTThreads / 64) {  // the wave's own columns
-        float *f = field + (long long)(k0 + col) * ld_grid;
-        for (int i = lane; i < N; i += 64) {
-            const float2 z = res[col * N + i];
-            f[rn + i] = z.x;
-            f[rs + i] = z.y;
+    fft_run<true>(b0, tw, as_global(P.fac) + G.j * kMaxFactors, as_global(P.nfac)[G.j], N, ncol, tid);
+    __syncthreads();
+    {
+        SHT_GLOBAL float *field = as_global(static_cast<float *>(C.out));
+        const long long ld = C.ld_out;
+        const long long rn = as_global(P.ring_n)[G.j], rs = as_global(P.ring_s)[G.j];
+        constexpr int U = (kFFTMaxRing + kFFTThreads - 1) / kFFTThreads;
+#pragma unroll
+        for (int c = 0; c < kFFTCols; ++c) {
+            if (c >= ncol) break;  // uniform
+            SHT_GLOBAL float *fn = field + (long long)(G.k0 + c) * ld + rn;
+            SHT_GLOBAL float *fs = field + (long long)(G.k0 + c) * ld + rs;
+            const float2 *z = b0 + c * N;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = u * kFFTThreads + tid;
+                if (i < N) {
+                    const float2 v = z[i];
+                    fn[(unsigned)i] = v.x;
+                    fs[(unsigned)i] = v.y;
+                }
+            }
         }
     }
 }
@@ -597,7 +616,7 @@ __global__ __launch_bounds__(kFFTThreads, 4) void fft_synthesis_kernel(const Dev
 // southern ring as one complex FFT of z = f_N + i f_S, separated by symmetry into f_N, f_S
 // and stored as (f_N + f_S, f_N - f_S) for orders 0..mt.
 template <bool BATCH>
-__global__ __launch_bounds__(kFFTThreads, 4) void fft_analysis_kernel(const DevPlan *__restrict__ plans,
+__global__ __launch_bounds__(kFFTThreads, 5) void fft_analysis_kernel(const DevPlan *__restrict__ plans,
                                                                    CallArgs single,
                                                                    const CallArgs *__restrict__ calls,
                                                                    const Tile *__restrict__ tiles) {
@@ -606,53 +625,57 @@ __global__ __launch_bounds__(kFFTThreads, 4) void fft_analysis_kernel(const DevP
     const Tile T = BATCH ? tiles[blockIdx.x] : Tile{0, 0, 0, 0};
     const CallArgs C = BATCH ? calls[T.call] : single;
     const DevPlan &P = plans[C.plan];
-    const float *__restrict__ field = static_cast<const float *>(C.in);
-    const long long ld_grid = C.ld_in;
-    float4 *__restrict__ fbuf = static_cast<float4 *>(C.fbuf);
     const int K = C.K;
-    const int j = BATCH ? T.m : P.ring_order[blockIdx.x];
-    const int N = P.nlon[j], mt = P.mt[j];
-    const int k0 = (BATCH ? T.blk : (int)blockIdx.y) * P.kg;
-    const int ncol = min(P.kg, K - k0);
+    const FFTTile G = fft_tile<BATCH>(P, T, K);
+    const int N = G.N, mt = G.mt, ncol = G.ncol;
     float2 *tw = smem;
     float2 *b0 = smem + N;
-    load_twiddles(P, j, N, tw, tid);
-    const long long rn = P.ring_n[j], rs = P.ring_s[j];
-    const int items = ncol * N;
-    const FastDiv divN(N);
-    constexpr int U = 8;
-    for (int i0 = 0; i0 < items; i0 += U * kFFTThreads) {
-        float2 v[U];
+    load_twiddles(P, G.j, N, tw, tid);
+    {
+        const SHT_GLOBAL float *field = as_global(static_cast<const float *>(C.in));
+        const long long ld = C.ld_in;
+        const long long rn = as_global(P.ring_n)[G.j], rs = as_global(P.ring_s)[G.j];
+        constexpr int U = (kFFTMaxRing + kFFTThreads - 1) / kFFTThreads;
+        float2 v[kFFTCols][U];
 #pragma unroll
-        for (int q = 0; q < U; ++q) {
-            const int it = i0 + q * kFFTThreads + tid;
-            if (it < items) {
-                const int col = divN.div(it), i = it - col * N;
-                const float *f = field + (long long)(k0 + col) * ld_grid;
-                v[q] = make_float2(f[rn + i], f[rs + i]);
+        for (int c = 0; c < kFFTCols; ++c) {
+            if (c >= ncol) break;  // uniform
+            const SHT_GLOBAL float *fn = field + (long long)(G.k0 + c) * ld + rn;
+            const SHT_GLOBAL float *fs = field + (long long)(G.k0 + c) * ld + rs;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = u * kFFTThreads + tid;
+                if (i < N) v[c][u] = make_float2(fn[(unsigned)i], fs[(unsigned)i]);
             }
         }
 #pragma unroll
-        for (int q = 0; q < U; ++q) {
-            const int it = i0 + q * kFFTThreads + tid;
-            if (it < items) b0[it] = v[q];
+        for (int c = 0; c < kFFTCols; ++c) {
+            if (c >= ncol) break;  // uniform
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = u * kFFTThreads + tid;
+                if (i < N) b0[c * N + i] = v[c][u];
+            }
         }
     }
     __syncthreads();
-    fft_run<false>(b0, tw, P.fac + j * kMaxFactors, P.nfac[j], N, ncol, tid);
+    fft_run<false>(b0, tw, as_global(P.fac) + G.j * kMaxFactors, as_global(P.nfac)[G.j], N, ncol, tid);
     __syncthreads();
-    const float2 *res = b0;
-    const long long fb0 = P.fboff[j];
-    const int outs = ncol * (mt + 1);
-    const FastDiv divC(ncol);
-    for (int it = tid; it < outs; it += kFFTThreads) {
-        const int mm = divC.div(it), col = it - mm * ncol;
-        const float2 p = res[col * N + mm];
-        const float2 q = res[col * N + (mm == 0 ? 0 : N - mm)];
-        // f_N = (Z_m + conj Z_{N-m})/2, f_S = (Z_m - conj Z_{N-m})/(2i)
-        const float nre = 0.5f * (p.x + q.x), nim = 0.5f * (p.y - q.y);
-        const float sre = 0.5f * (p.y + q.y), sim = 0.5f * (q.x - p.x);
-        fbuf[(fb0 + mm) * (long long)K + k0 + col] = make_float4(nre + sre, nim + sim, nre - sre, nim - sim);
+    {
+        SHT_GLOBAL float4 *fb = as_global(static_cast<float4 *>(C.fbuf)) + as_global(P.fboff)[G.j] * (long long)K + G.k0;
+        const int col = tid % kFFTCols, m0 = tid / kFFTCols;
+        const float2 *z = b0 + col * N;
+#pragma unroll
+        for (int u = 0; u < kFFTRowPasses; ++u) {
+            const int mm = m0 + kFFTRowsPerPass * u;
+            if (col >= ncol || mm > mt) continue;
+            const float2 p = z[mm];
+            const float2 q = z[mm == 0 ? 0 : N - mm];
+            // f_N = (Z_m + conj Z_{N-m})/2, f_S = (Z_m - conj Z_{N-m})/(2i)
+            const float nre = 0.5f * (p.x + q.x), nim = 0.5f * (p.y - q.y);
+            const float sre = 0.5f * (p.y + q.y), sim = 0.5f * (q.x - p.x);
+            gstore(fb + (unsigned)(mm * K + col), make_float4(nre + sre, nim + sim, nre - sre, nim - sim));
+        }
     }
 }
 

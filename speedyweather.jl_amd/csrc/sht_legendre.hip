@@ -131,16 +131,16 @@ __global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *
                                                                  const CallArgs *__restrict__ calls,
                                                                  const Tile *__restrict__ tiles) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const Tile T = tiles[blockIdx.x];
+    const Tile T = tiles[BATCH ? blockIdx.x : blockIdx.y];
     const CallArgs C = BATCH ? calls[T.call] : single;
     const DevPlan &P = plans[C.plan];
-    const int m = T.m, cg = BATCH ? T.cg : (int)blockIdx.y;
-    const float2 *__restrict__ coeffs = static_cast<const float2 *>(C.in);
+    const int m = T.m, cg = BATCH ? T.cg : (int)blockIdx.x;
+    const SHT_GLOBAL float2 *coeffs = as_global(static_cast<const float2 *>(C.in));
     const long long ld_spec = C.ld_in;
-    float4 *__restrict__ fbuf = static_cast<float4 *>(C.fbuf);
+    SHT_GLOBAL float4 *fbuf = as_global(static_cast<float4 *>(C.fbuf));
     const int K = C.K, unscale = C.unscale;
     const int n = P.n;
-    const int j0 = P.j0[m];
+    const int j0 = as_global(P.j0)[m];
     const int J = n - j0;
     const int L = P.lmax - m;
     const int Le = (L + 1) >> 1;
@@ -148,13 +148,13 @@ __global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *
     const int Jt = (J + 15) >> 4;
     const int tile = T.blk * 4 + w;
     const bool active = tile < Jt;
-    const long long off = P.lm_off[m];
+    const long long off = as_global(P.lm_off)[m];
     const int kbase = cg * kColsComplex;
     const int row = lane & 15, grp = lane >> 4;
     // staging role
     const int kk = tid >> 3, g = tid & 7;
     const bool kok = kbase + kk < K;
-    const float2 *csrc = coeffs + (long long)(kok ? kbase + kk : 0) * ld_spec + off;
+    const SHT_GLOBAL float2 *csrc = coeffs + (long long)(kok ? kbase + kk : 0) * ld_spec + off;
 
     __shared__ alignas(16) unsigned char sm[LegSmem<MODE>::bytes];
 
@@ -170,7 +170,8 @@ __global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int o = 64 * c + 8 * g + e;
-            v[e] = (kok && o < L) ? csrc[o] : make_float2(0.f, 0.f);
+            v[e] = make_float2(0.f, 0.f);
+            if (kok && o < L) v[e] = gload(csrc + o);
         }
     };
     float af[2][8], afn[2][8];
@@ -178,16 +179,19 @@ __global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *
     auto load_lambda = [&](int c, float (&f)[2][8], V8of<MODE> (&h)[2], V8of<MODE> (&lo)[2]) {
         if (!active) return;
         if constexpr (MODE == kFP32) {
-            const float *A = P.syn_f32 + P.syn_off[m];
+            const SHT_GLOBAL float *A = as_global(P.syn_f32) + as_global(P.syn_off)[m];
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
-                const float *Ap = A + ((long long)(p * Jt + tile) * (nch * kChunk) + c * kChunk) * 16;
+                const SHT_GLOBAL float *Ap = A + ((long long)(p * Jt + tile) * (nch * kChunk) + c * kChunk) * 16;
 #pragma unroll
                 for (int ks = 0; ks < kChunk / 4; ++ks) f[p][ks] = Ap[(ks * 4 + grp) * 16 + row];
             }
         } else {
-            const V8of<MODE> *Ah = reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.syn_hi) + P.syn_off[m]);
-            const V8of<MODE> *Al = reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.syn_lo) + P.syn_off[m]);
+            const long long so = as_global(P.syn_off)[m];
+            const SHT_GLOBAL V8of<MODE> *Ah =
+                as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.syn_hi) + so));
+            const SHT_GLOBAL V8of<MODE> *Al =
+                as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.syn_lo) + so));
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
                 const long long fi = ((long long)(p * Jt + tile) * nch + c) * 64 + lane;
@@ -240,24 +244,25 @@ __global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *
             const int j = j0 + 16 * tile + r;
             float e = acc[0][cb][i], o = acc[1][cb][i];
             if constexpr (MODE == kFP16x3) {
-                const float sa = P.syn_sinv[P.syn_soff[m] + 16 * tile + r];
+                const float sa = as_global(P.syn_sinv)[as_global(P.syn_soff)[m] + 16 * tile + r];
                 e *= sa;
                 o *= sa;
             }
             float nv = e + o, sv = e - o;
             if (unscale && j < n) {
-                const float ci = P.coslat_inv[j];
+                const float ci = as_global(P.coslat_inv)[j];
                 nv *= ci;
                 sv *= ci;
             }
             const float n_other = __shfl_xor(nv, 1);
             const float s_other = __shfl_xor(sv, 1);
             if (j < n && k < K) {
-                float *dst = reinterpret_cast<float *>(fbuf + (P.fboff[j] + m) * (long long)K + k);
+                SHT_GLOBAL float2 *dst =
+                    reinterpret_cast<SHT_GLOBAL float2 *>(fbuf + (as_global(P.fboff)[j] + m) * (long long)K + k);
                 if (reim == 0) {
-                    *reinterpret_cast<float2 *>(dst) = make_float2(nv, n_other);
+                    gstore(dst, make_float2(nv, n_other));
                 } else {
-                    *reinterpret_cast<float2 *>(dst + 2) = make_float2(s_other, sv);
+                    gstore(dst + 1, make_float2(s_other, sv));
                 }
             }
         }
@@ -277,16 +282,16 @@ __global__ __launch_bounds__(256) void legendre_analysis_kernel(const DevPlan *_
                                                                 const CallArgs *__restrict__ calls,
                                                                 const Tile *__restrict__ tiles) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const Tile T = tiles[blockIdx.x];
+    const Tile T = tiles[BATCH ? blockIdx.x : blockIdx.y];
     const CallArgs C = BATCH ? calls[T.call] : single;
     const DevPlan &P = plans[C.plan];
-    const int m = T.m, cg = BATCH ? T.cg : (int)blockIdx.y;
-    const float4 *__restrict__ fbuf = static_cast<const float4 *>(C.fbuf);
-    float2 *__restrict__ coeffs = static_cast<float2 *>(C.out);
+    const int m = T.m, cg = BATCH ? T.cg : (int)blockIdx.x;
+    const SHT_GLOBAL float4 *fbuf = as_global(static_cast<const float4 *>(C.fbuf));
+    SHT_GLOBAL float2 *coeffs = as_global(static_cast<float2 *>(C.out));
     const long long ld_spec = C.ld_out;
     const int K = C.K;
     const int n = P.n;
-    const int j0 = P.j0[m];
+    const int j0 = as_global(P.j0)[m];
     const int J = n - j0;
     const int L = P.lmax - m;
     const int NG = (L + kDegPerWG - 1) / kDegPerWG;
@@ -294,7 +299,7 @@ __global__ __launch_bounds__(256) void legendre_analysis_kernel(const DevPlan *_
     const int JC = (J + kChunk - 1) / kChunk;
     const int p = w & 1;
     const int lt = 2 * T.blk + (w >> 1);
-    const long long off = P.lm_off[m];
+    const long long off = as_global(P.lm_off)[m];
     const int kbase = cg * kColsComplex;
     const int row = lane & 15, grp = lane >> 4;
     const int kk = tid >> 3, g = tid & 7;
@@ -314,19 +319,24 @@ __global__ __launch_bounds__(256) void legendre_analysis_kernel(const DevPlan *_
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int j = j0 + kChunk * c + 4 * g + i;
-            v[i] = (kok && j < n) ? fbuf[(P.fboff[j] + m) * (long long)K + kgl] : make_float4(0.f, 0.f, 0.f, 0.f);
+            v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (kok && j < n) v[i] = gload(fbuf + (as_global(P.fboff)[j] + m) * (long long)K + kgl);
         }
     };
     float af[8], afn[8];
     V8of<MODE> ah, al, ahn, aln;
     auto load_lambda = [&](int c, float (&f)[8], V8of<MODE> &h, V8of<MODE> &lo) {
         if constexpr (MODE == kFP32) {
-            const float *A = P.ana_f32 + P.ana_off[m] + ((long long)(p * nt + lt) * (JC * kChunk) + c * kChunk) * 16;
+            const SHT_GLOBAL float *A = as_global(P.ana_f32) + as_global(P.ana_off)[m] +
+                                        ((long long)(p * nt + lt) * (JC * kChunk) + c * kChunk) * 16;
 #pragma unroll
             for (int ks = 0; ks < kChunk / 4; ++ks) f[ks] = A[(ks * 4 + grp) * 16 + row];
         } else {
-            const V8of<MODE> *Ah = reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_hi) + P.ana_off[m]);
-            const V8of<MODE> *Al = reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_lo) + P.ana_off[m]);
+            const long long ao = as_global(P.ana_off)[m];
+            const SHT_GLOBAL V8of<MODE> *Ah =
+                as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_hi) + ao));
+            const SHT_GLOBAL V8of<MODE> *Al =
+                as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_lo) + ao));
             const long long fi = ((long long)(p * nt + lt) * JC + c) * 64 + lane;
             h = Ah[fi];
             lo = Al[fi];
@@ -375,7 +385,7 @@ __global__ __launch_bounds__(256) void legendre_analysis_kernel(const DevPlan *_
             const int r = 4 * grp + i;
             const int lq = 16 * (w >> 1) + r;  // l' - 32 blk
             float val = acc[cb][i];
-            if constexpr (MODE == kFP16x3) val *= P.ana_sinv[P.ana_soff[m] + (p * nt + lt) * 16 + r];
+            if constexpr (MODE == kFP16x3) val *= as_global(P.ana_sinv)[as_global(P.ana_soff)[m] + (p * nt + lt) * 16 + r];
             float *o = reinterpret_cast<float *>(&Out[kc * (kDegPerWG + 1) + 2 * lq + p]);
             o[reim] = val;
         }
@@ -386,7 +396,7 @@ __global__ __launch_bounds__(256) void legendre_analysis_kernel(const DevPlan *_
         const int idx = it * 256 + tid;
         const int kc = idx >> 6, o = idx & 63;
         const int og = kDegPerWG * T.blk + o, kq = kbase + kc;
-        if (og < L && kq < K) coeffs[(long long)kq * ld_spec + off + og] = Out[kc * (kDegPerWG + 1) + o];
+        if (og < L && kq < K) gstore(coeffs + (long long)kq * ld_spec + off + og, Out[kc * (kDegPerWG + 1) + o]);
     }
 }
 

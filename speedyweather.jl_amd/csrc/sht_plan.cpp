@@ -696,13 +696,13 @@ int sht_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *fie
     hipError_t e;
     {
         ProfScope ps(plan, SHT_KERNEL_LEGENDRE_SYNTHESIS, stream);
-        e = launch_stage(sht::legendre_kernel(sht::kLegendreSynthesis, plan->d.precision, false), dim3(plan->nsyn, cgs),
+        e = launch_stage(sht::legendre_kernel(sht::kLegendreSynthesis, plan->d.precision, false), dim3(cgs, plan->nsyn),
                          0, plan->dP, c, nullptr, plan->syn_tiles, stream);
     }
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("legendre synthesis launch: ") + hipGetErrorString(e));
     {
         ProfScope ps(plan, SHT_KERNEL_FFT_SYNTHESIS, stream);
-        e = launch_stage(sht::fft_kernel(sht::kFFTSynthesis, false), dim3(plan->g.n, (K + plan->P.kg - 1) / plan->P.kg),
+        e = launch_stage(sht::fft_kernel(sht::kFFTSynthesis, false), dim3((K + plan->P.kg - 1) / plan->P.kg, plan->g.n),
                          plan->P.fft_smem, plan->dP, c, nullptr, nullptr, stream);
     }
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("fft synthesis launch: ") + hipGetErrorString(e));
@@ -721,13 +721,13 @@ int sht_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void *coeff
     hipError_t e;
     {
         ProfScope ps(plan, SHT_KERNEL_FFT_ANALYSIS, stream);
-        e = launch_stage(sht::fft_kernel(sht::kFFTAnalysis, false), dim3(plan->g.n, (K + plan->P.kg - 1) / plan->P.kg),
+        e = launch_stage(sht::fft_kernel(sht::kFFTAnalysis, false), dim3((K + plan->P.kg - 1) / plan->P.kg, plan->g.n),
                          plan->P.fft_smem, plan->dP, c, nullptr, nullptr, stream);
     }
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("fft analysis launch: ") + hipGetErrorString(e));
     {
         ProfScope ps(plan, SHT_KERNEL_LEGENDRE_ANALYSIS, stream);
-        e = launch_stage(sht::legendre_kernel(sht::kLegendreAnalysis, plan->d.precision, false), dim3(plan->nana, cgs),
+        e = launch_stage(sht::legendre_kernel(sht::kLegendreAnalysis, plan->d.precision, false), dim3(cgs, plan->nana),
                          0, plan->dP, c, nullptr, plan->ana_tiles, stream);
     }
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("legendre analysis launch: ") + hipGetErrorString(e));
