@@ -511,6 +511,7 @@ __device__ inline void load_twiddles(const DevPlan &P, int j, int N, float2 *tw,
         const int i = q * kFFTThreads + tid;
         if (i < N) v[q] = gload(gtw + i);
     }
+    __builtin_amdgcn_sched_barrier(0);  // every load in flight before the first LDS store
 #pragma unroll
     for (int q = 0; q < U; ++q) {
         const int i = q * kFFTThreads + tid;
@@ -523,15 +524,41 @@ struct FFTTile {
     int j, N, mt, k0, ncol;
 };
 
-template <bool BATCH>
-__device__ inline FFTTile fft_tile(const DevPlan &P, const Tile &T, int K) {
-    FFTTile t;
-    t.j = BATCH ? T.m : as_global(P.ring_order)[blockIdx.y];
+// Single-call launch orders, rings heaviest first.  PAIRED (synthesis, which reads the
+// Fourier rows): units (ring rank r, column-group pair p) go round-robin to the XCDs and
+// the two column groups of a unit, which share the 128-byte lines of the Fourier rows, run
+// back to back on the same XCD: block b -> XCD b % 8, slot s = b / 8, unit
+// (s / 2) * 8 + b % 8, column group 2 p + s % 2.  Otherwise (analysis): block b -> ring
+// rank b / kgs, column group b % kgs.  Returns false for blocks past the end.
+template <bool BATCH, bool PAIRED>
+__device__ inline bool fft_tile(const DevPlan &P, const Tile &T, int K, FFTTile &t) {
+    const int kgs = (K + kFFTCols - 1) / kFFTCols;
+    int cg;
+    if (BATCH) {
+        if (T.m < 0) return false;
+        t.j = T.m;
+        cg = T.blk;
+    } else {
+        const int b = blockIdx.x;
+        int r;
+        if (PAIRED) {
+            const int npair = (kgs + 1) >> 1;
+            const int s = b / kXCDs;
+            const int u = (s >> 1) * kXCDs + b % kXCDs;
+            r = u / npair;
+            cg = 2 * (u - r * npair) + (s & 1);
+        } else {
+            r = b / kgs;
+            cg = b - r * kgs;
+        }
+        if (r >= P.n || cg >= kgs) return false;
+        t.j = as_global(P.ring_order)[r];
+    }
     t.N = as_global(P.nlon)[t.j];
     t.mt = as_global(P.mt)[t.j];
-    t.k0 = (BATCH ? T.blk : (int)blockIdx.x) * kFFTCols;
+    t.k0 = cg * kFFTCols;
     t.ncol = min(kFFTCols, K - t.k0);
-    return t;
+    return true;
 }
 
 // Fourier rows are read and written with thread -> (order mm = tid/4 + 64 u, column tid%4).
@@ -552,7 +579,8 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_synthesis_kernel(const Dev
     const CallArgs C = BATCH ? calls[T.call] : single;
     const DevPlan &P = plans[C.plan];
     const int K = C.K;
-    const FFTTile G = fft_tile<BATCH>(P, T, K);
+    FFTTile G;
+    if (!fft_tile<BATCH, true>(P, T, K, G)) return;  // padding of the launch order
     const int N = G.N, mt = G.mt, ncol = G.ncol;
     float2 *tw = smem;
     float2 *b0 = smem + N;
@@ -569,6 +597,7 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_synthesis_kernel(const Dev
             v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (col < ncol && mm <= mt) v[u] = gload(fb + (unsigned)(mm * K + col));
         }
+        __builtin_amdgcn_sched_barrier(0);  // every load in flight before the first LDS store
         float2 *z = b0 + col * N;
 #pragma unroll
         for (int u = 0; u < kFFTRowPasses; ++u) {
@@ -626,7 +655,8 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_analysis_kernel(const DevP
     const CallArgs C = BATCH ? calls[T.call] : single;
     const DevPlan &P = plans[C.plan];
     const int K = C.K;
-    const FFTTile G = fft_tile<BATCH>(P, T, K);
+    FFTTile G;
+    if (!fft_tile<BATCH, false>(P, T, K, G)) return;  // padding of the launch order
     const int N = G.N, mt = G.mt, ncol = G.ncol;
     float2 *tw = smem;
     float2 *b0 = smem + N;
@@ -636,25 +666,28 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_analysis_kernel(const DevP
         const long long ld = C.ld_in;
         const long long rn = as_global(P.ring_n)[G.j], rs = as_global(P.ring_s)[G.j];
         constexpr int U = (kFFTMaxRing + kFFTThreads - 1) / kFFTThreads;
-        float2 v[kFFTCols][U];
+        // all loads of the thread in flight before the first LDS store (columns past ncol
+        // re-read column 0 and are not stored)
+        float vn[kFFTCols][U], vs[kFFTCols][U];
 #pragma unroll
         for (int c = 0; c < kFFTCols; ++c) {
-            if (c >= ncol) break;  // uniform
-            const SHT_GLOBAL float *fn = field + (long long)(G.k0 + c) * ld + rn;
-            const SHT_GLOBAL float *fs = field + (long long)(G.k0 + c) * ld + rs;
+            const int cc = c < ncol ? c : 0;
+            const SHT_GLOBAL float *fn = field + (long long)(G.k0 + cc) * ld + rn;
+            const SHT_GLOBAL float *fs = field + (long long)(G.k0 + cc) * ld + rs;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int i = u * kFFTThreads + tid;
-                if (i < N) v[c][u] = make_float2(fn[(unsigned)i], fs[(unsigned)i]);
+                const unsigned i = min(u * kFFTThreads + tid, N - 1);
+                vn[c][u] = fn[i];
+                vs[c][u] = fs[i];
             }
         }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int c = 0; c < kFFTCols; ++c) {
-            if (c >= ncol) break;  // uniform
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int i = u * kFFTThreads + tid;
-                if (i < N) b0[c * N + i] = v[c][u];
+                if (c < ncol && i < N) b0[c * N + i] = make_float2(vn[c][u], vs[c][u]);
             }
         }
     }

@@ -38,10 +38,15 @@ enum Mode { kFP32 = 0, kFP16x3 = 1, kBF16x3 = 2 };
 // indexes the launch's call arguments (always 0 for a single transform).
 struct Tile {
     int call;
-    int m;
+    int m;  // < 0: padding, the workgroup exits at once
     int blk;
     int cg;
 };
+
+// Workgroups go to the 8 XCDs (each with its own L2) round-robin by linear block index.
+// Launch orders place workgroups that read the same data (the tiles of one order m; the
+// two FFT column groups that share 128-byte Fourier lines) on one XCD, consecutively.
+constexpr int kXCDs = 8;
 
 // Arguments of one transform call inside a launch.
 struct CallArgs {
@@ -58,7 +63,7 @@ struct CallArgs {
 
 enum Stage { kLegendreSynthesis = 0, kFFTSynthesis = 1, kFFTAnalysis = 2, kLegendreAnalysis = 3 };
 
-// Device view of a plan, passed by value to every kernel.
+// Device view of a plan (kernels read it through a device pointer).
 struct DevPlan {
     int n;            // rings per hemisphere
     int lmax, mmax;   // 1-based
@@ -114,8 +119,9 @@ void set_last_error(const std::string &msg);
 
 // Kernel entry points.  Every stage kernel takes
 //   (const DevPlan *plans, CallArgs single, const CallArgs *calls, const Tile *tiles)
-// and, for batch = false, reads the single call's arguments by value and its column group
-// from blockIdx.y; for batch = true, the call and column group of each tile.
+// on a 1-D grid.  batch = false: the single call's arguments by value; the Legendre
+// kernels read their tile from tiles[blockIdx.x], the FFT kernels derive ring and column
+// group from blockIdx.x.  batch = true: tile, call and column group from tiles.
 const void *legendre_kernel(int stage, int mode, bool batch);
 const void *fft_kernel(int stage, bool batch);
 int launch_uv_from_vordiv(const DevPlan *P, int nharm, const void *vor, const void *div, void *U,

@@ -131,10 +131,11 @@ __global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *
                                                                  const CallArgs *__restrict__ calls,
                                                                  const Tile *__restrict__ tiles) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const Tile T = tiles[BATCH ? blockIdx.x : blockIdx.y];
+    const Tile T = tiles[blockIdx.x];
+    if (T.m < 0) return;  // padding of the launch order
     const CallArgs C = BATCH ? calls[T.call] : single;
     const DevPlan &P = plans[C.plan];
-    const int m = T.m, cg = BATCH ? T.cg : (int)blockIdx.x;
+    const int m = T.m, cg = T.cg;
     const SHT_GLOBAL float2 *coeffs = as_global(static_cast<const float2 *>(C.in));
     const long long ld_spec = C.ld_in;
     SHT_GLOBAL float4 *fbuf = as_global(static_cast<float4 *>(C.fbuf));
@@ -282,10 +283,11 @@ __global__ __launch_bounds__(256) void legendre_analysis_kernel(const DevPlan *_
                                                                 const CallArgs *__restrict__ calls,
                                                                 const Tile *__restrict__ tiles) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const Tile T = tiles[BATCH ? blockIdx.x : blockIdx.y];
+    const Tile T = tiles[blockIdx.x];
+    if (T.m < 0) return;  // padding of the launch order
     const CallArgs C = BATCH ? calls[T.call] : single;
     const DevPlan &P = plans[C.plan];
-    const int m = T.m, cg = BATCH ? T.cg : (int)blockIdx.x;
+    const int m = T.m, cg = T.cg;
     const SHT_GLOBAL float4 *fbuf = as_global(static_cast<const float4 *>(C.fbuf));
     SHT_GLOBAL float2 *coeffs = as_global(static_cast<float2 *>(C.out));
     const long long ld_spec = C.ld_out;

@@ -219,15 +219,63 @@ struct Profile {
     std::vector<hipEvent_t> ev;  // [(kernel * max_calls + call) * 2 + {start, stop}]
 };
 
+// A launch order for workgroups grouped in units that share data (tiles of one unit run
+// back to back on one XCD): units, heaviest first, go greedily to the least-loaded XCD;
+// block b then takes slot b / 8 of XCD b % 8's list.  Lists are padded to equal length
+// with padding tiles (m = -1).
+struct LaunchUnit {
+    double cost;
+    std::vector<Tile> tiles;
+};
+
+static std::vector<Tile> xcd_order(std::vector<LaunchUnit> units) {
+    std::stable_sort(units.begin(), units.end(),
+                     [](const LaunchUnit &a, const LaunchUnit &b) { return a.cost > b.cost; });
+    std::vector<std::vector<Tile>> lists(sht::kXCDs);
+    std::vector<double> load(sht::kXCDs, 0.0);
+    for (const LaunchUnit &u : units) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[x] += u.cost;
+        lists[x].insert(lists[x].end(), u.tiles.begin(), u.tiles.end());
+    }
+    size_t len = 0;
+    for (const auto &l : lists) len = std::max(len, l.size());
+    std::vector<Tile> out(len * sht::kXCDs, Tile{0, -1, 0, 0});
+    for (int x = 0; x < sht::kXCDs; ++x)
+        for (size_t q = 0; q < lists[x].size(); ++q) out[q * sht::kXCDs + x] = lists[x][q];
+    return out;
+}
+
+// Legendre units: the tiles of one order m (all ring or degree groups, all column groups).
+static std::vector<LaunchUnit> legendre_units(const std::vector<Tile> &t, const std::vector<double> &c, int call,
+                                              int cgs) {
+    std::vector<LaunchUnit> units;
+    std::vector<int> unit_of_m;
+    for (size_t q = 0; q < t.size(); ++q) {
+        const int m = t[q].m;
+        if (m >= (int)unit_of_m.size()) unit_of_m.resize(m + 1, -1);
+        if (unit_of_m[m] < 0) {
+            unit_of_m[m] = (int)units.size();
+            units.push_back({0.0, {}});
+        }
+        LaunchUnit &u = units[unit_of_m[m]];
+        for (int cg = 0; cg < cgs; ++cg) {
+            u.tiles.push_back({call, m, t[q].blk, cg});
+            u.cost += c[q];
+        }
+    }
+    return units;
+}
+
 struct sht_plan {
     sht_desc d{};
     Geometry g;
     DevPlan P{};
     const DevPlan *dP = nullptr;  // device copy of P
     std::vector<void *> allocs;
-    const Tile *syn_tiles = nullptr;
-    const Tile *ana_tiles = nullptr;
-    int nsyn = 0, nana = 0;
+    // single-call Legendre launch orders, one per column-group count cgs = 1..max
+    std::vector<const Tile *> syn_tiles, ana_tiles;
+    std::vector<int> nsyn, nana;
     // host copies for batches: Legendre tiles (heavy first) with their costs, FFT ring order
     std::vector<Tile> h_syn, h_ana;
     std::vector<double> h_syn_cost, h_ana_cost, h_ring_cost;
@@ -575,10 +623,20 @@ int build_plan(sht_plan *pl) {
     if ((rc = dev_upload(A, uv1, &P.uv_1))) return rc;
     if ((rc = dev_upload(A, uv2, &P.uv_2))) return rc;
     if ((rc = dev_upload(A, l_of, &P.l_of))) return rc;
-    if ((rc = dev_upload(A, syn_t, &pl->syn_tiles))) return rc;
-    if ((rc = dev_upload(A, ana_t, &pl->ana_tiles))) return rc;
-    pl->nsyn = (int)syn_t.size();
-    pl->nana = (int)ana_t.size();
+    {
+        const int cgs_max = (d.nlayers + sht::kColsComplex - 1) / sht::kColsComplex;
+        for (int cgs = 1; cgs <= cgs_max; ++cgs) {
+            const std::vector<Tile> so = xcd_order(legendre_units(syn_t, syn_c, 0, cgs));
+            const std::vector<Tile> ao = xcd_order(legendre_units(ana_t, ana_c, 0, cgs));
+            const Tile *ds = nullptr, *da = nullptr;
+            if ((rc = dev_upload(A, so, &ds))) return rc;
+            if ((rc = dev_upload(A, ao, &da))) return rc;
+            pl->syn_tiles.push_back(ds);
+            pl->ana_tiles.push_back(da);
+            pl->nsyn.push_back((int)so.size());
+            pl->nana.push_back((int)ao.size());
+        }
+    }
     {
         std::vector<DevPlan> one(1, P);
         if ((rc = dev_upload(A, one, &pl->dP))) return rc;
@@ -664,6 +722,16 @@ int sht_plan_destroy(sht_plan *plan) {
     return SHT_OK;
 }
 
+// Blocks of a single-call FFT launch (the kernels map block -> ring, column group):
+// paired, units (ring, pair of column groups) of two blocks round-robin over the XCDs;
+// otherwise one block per (ring, column group).
+static unsigned fft_blocks(int nrings, int K, bool paired) {
+    const int kgs = (K + sht::kFFTCols - 1) / sht::kFFTCols;
+    if (!paired) return (unsigned)(nrings * kgs);
+    const long long units = (long long)nrings * ((kgs + 1) / 2);
+    return (unsigned)(2 * sht::kXCDs * ((units + sht::kXCDs - 1) / sht::kXCDs));
+}
+
 // Launches one stage kernel (every stage kernel has the same signature).
 static hipError_t launch_stage(const void *func, dim3 grid, unsigned smem, const DevPlan *plans, const CallArgs &c,
                                const CallArgs *calls, const Tile *tiles, void *stream) {
@@ -696,13 +764,13 @@ int sht_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *fie
     hipError_t e;
     {
         ProfScope ps(plan, SHT_KERNEL_LEGENDRE_SYNTHESIS, stream);
-        e = launch_stage(sht::legendre_kernel(sht::kLegendreSynthesis, plan->d.precision, false), dim3(cgs, plan->nsyn),
-                         0, plan->dP, c, nullptr, plan->syn_tiles, stream);
+        e = launch_stage(sht::legendre_kernel(sht::kLegendreSynthesis, plan->d.precision, false),
+                         dim3(plan->nsyn[cgs - 1]), 0, plan->dP, c, nullptr, plan->syn_tiles[cgs - 1], stream);
     }
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("legendre synthesis launch: ") + hipGetErrorString(e));
     {
         ProfScope ps(plan, SHT_KERNEL_FFT_SYNTHESIS, stream);
-        e = launch_stage(sht::fft_kernel(sht::kFFTSynthesis, false), dim3((K + plan->P.kg - 1) / plan->P.kg, plan->g.n),
+        e = launch_stage(sht::fft_kernel(sht::kFFTSynthesis, false), dim3(fft_blocks(plan->g.n, K, true)),
                          plan->P.fft_smem, plan->dP, c, nullptr, nullptr, stream);
     }
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("fft synthesis launch: ") + hipGetErrorString(e));
@@ -721,14 +789,14 @@ int sht_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void *coeff
     hipError_t e;
     {
         ProfScope ps(plan, SHT_KERNEL_FFT_ANALYSIS, stream);
-        e = launch_stage(sht::fft_kernel(sht::kFFTAnalysis, false), dim3((K + plan->P.kg - 1) / plan->P.kg, plan->g.n),
+        e = launch_stage(sht::fft_kernel(sht::kFFTAnalysis, false), dim3(fft_blocks(plan->g.n, K, false)),
                          plan->P.fft_smem, plan->dP, c, nullptr, nullptr, stream);
     }
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("fft analysis launch: ") + hipGetErrorString(e));
     {
         ProfScope ps(plan, SHT_KERNEL_LEGENDRE_ANALYSIS, stream);
-        e = launch_stage(sht::legendre_kernel(sht::kLegendreAnalysis, plan->d.precision, false), dim3(cgs, plan->nana),
-                         0, plan->dP, c, nullptr, plan->ana_tiles, stream);
+        e = launch_stage(sht::legendre_kernel(sht::kLegendreAnalysis, plan->d.precision, false),
+                         dim3(plan->nana[cgs - 1]), 0, plan->dP, c, nullptr, plan->ana_tiles[cgs - 1], stream);
     }
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("legendre analysis launch: ") + hipGetErrorString(e));
     return SHT_OK;
@@ -843,8 +911,7 @@ const void *stage_func(const sht_batch *b, int st) {
 int build_batch(sht_batch *b, const sht_batch_item *items, int n, int flags) {
     std::vector<sht_plan *> plans;
     std::vector<CallArgs> calls;
-    std::vector<Tile> st_tiles[4];
-    std::vector<double> st_cost[4];
+    std::vector<LaunchUnit> st_units[4];
     int rc;
     // scratch: one Fourier buffer per item (a round trip's two calls share it)
     std::vector<size_t> fb_off(n);
@@ -869,17 +936,18 @@ int build_batch(sht_batch *b, const sht_batch_item *items, int n, int flags) {
         const int kgs = (K + pl->P.kg - 1) / pl->P.kg;
         auto add_fft = [&](int st, int call) {
             for (int j : pl->h_ring_order)
-                for (int q = 0; q < kgs; ++q) {
-                    st_tiles[st].push_back({call, j, q, 0});
-                    st_cost[st].push_back(pl->h_ring_cost[j]);
+                for (int q = 0; q < kgs; q += 2) {
+                    LaunchUnit u{0.0, {}};
+                    for (int h = q; h < std::min(q + 2, kgs); ++h) {
+                        u.tiles.push_back({call, j, h, 0});
+                        u.cost += pl->h_ring_cost[j];
+                    }
+                    st_units[st].push_back(u);
                 }
         };
         auto add_leg = [&](int st, int call, const std::vector<Tile> &t, const std::vector<double> &c) {
-            for (size_t q = 0; q < t.size(); ++q)
-                for (int cg = 0; cg < cgs; ++cg) {
-                    st_tiles[st].push_back({call, t[q].m, t[q].blk, cg});
-                    st_cost[st].push_back(c[q]);
-                }
+            const std::vector<LaunchUnit> u = legendre_units(t, c, call, cgs);
+            st_units[st].insert(st_units[st].end(), u.begin(), u.end());
         };
         if (it.direction == SHT_DIR_SYNTHESIS || it.direction == SHT_DIR_ROUNDTRIP) {
             void *grid = it.direction == SHT_DIR_ROUNDTRIP ? it.mid : it.out;
@@ -906,14 +974,10 @@ int build_batch(sht_batch *b, const sht_batch_item *items, int n, int flags) {
     if ((rc = dev_upload(b->allocs, dplans, &b->plans))) return rc;
     if ((rc = dev_upload(b->allocs, calls, &b->calls))) return rc;
     for (int st = 0; st < 4; ++st) {
-        // heaviest work first across all items
-        std::vector<size_t> idx(st_tiles[st].size());
-        std::iota(idx.begin(), idx.end(), 0);
-        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t c) { return st_cost[st][a] > st_cost[st][c]; });
-        std::vector<Tile> sorted(idx.size());
-        for (size_t q = 0; q < idx.size(); ++q) sorted[q] = st_tiles[st][idx[q]];
-        b->ntiles[st] = (int)sorted.size();
-        if ((rc = dev_upload(b->allocs, sorted, &b->tiles[st]))) return rc;
+        // heaviest units first across all items, spread over the XCDs
+        const std::vector<Tile> order = st_units[st].empty() ? std::vector<Tile>() : xcd_order(st_units[st]);
+        b->ntiles[st] = (int)order.size();
+        if (!order.empty() && (rc = dev_upload(b->allocs, order, &b->tiles[st]))) return rc;
     }
     if (b->fft_smem > 65536) {
         for (int st : {sht::kFFTSynthesis, sht::kFFTAnalysis})
