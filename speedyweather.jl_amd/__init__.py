@@ -3,12 +3,13 @@ SpeedyTransforms' API over libsht's C ABI; hand-written gfx950 kernels underneat
 from ._lib import DimensionMismatch, SHTError, lib
 from .grids import (Field, FullGaussianGrid, OctahedralGaussianGrid, get_nlat_half, get_truncation,
                     roundup_fft)
+from .mlp import ColumnMLP
 from .spectrum import LowerTriangularArray, Spectrum
 from .synthetic import seeded_grid, seeded_spectrum
 from .transform import SpectralBatch, SpectralTransform, UV_from_vordiv_, transform, transform_
 
 __all__ = [
-    "DimensionMismatch", "SHTError", "lib", "Field", "FullGaussianGrid", "OctahedralGaussianGrid",
+    "ColumnMLP", "DimensionMismatch", "SHTError", "lib", "Field", "FullGaussianGrid", "OctahedralGaussianGrid",
     "get_nlat_half", "get_truncation", "roundup_fft", "LowerTriangularArray", "Spectrum",
     "seeded_grid", "seeded_spectrum", "SpectralBatch", "SpectralTransform", "UV_from_vordiv_", "transform",
     "transform_",
