@@ -3,6 +3,7 @@ SpeedyTransforms' API over libsht's C ABI; hand-written gfx950 kernels underneat
 from ._lib import DimensionMismatch, SHTError, lib
 from .grids import (Field, FullGaussianGrid, OctahedralGaussianGrid, get_nlat_half, get_truncation,
                     roundup_fft)
+from . import dycore, learned
 from .mlp import ColumnMLP
 from .spectrum import LowerTriangularArray, Spectrum
 from .synthetic import seeded_grid, seeded_spectrum
