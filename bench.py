@@ -160,6 +160,44 @@ def bench_c4(sw, torch, steps, warmup):
     return res
 
 
+def bench_c5(sw, seeds=40, heldout=8, steps=3000):
+    """Dry primitive-equation step at T31 L8 with the learned column correction: data from
+    T127 L8 runs (zonal jets, N(0, 1 K) noise), MLP training on the engine's kernels, and the
+    A/B scores of the corrected coarse model against the plain one on held-out runs."""
+    import torch
+    ln, dy = sw.learned, sw.dycore
+    t0 = time.perf_counter()
+    x, y = ln.generate(list(range(seeds)), nsamples=2)
+    xv, yv = ln.generate(list(range(10000, 10000 + heldout)), nsamples=2)
+    t_gen = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    net, nx, ny, losses = ln.fit(x, y, steps=steps, batch=4096)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t0
+    held = ln.ab_score(net, nx, ny, xv, yv)
+    fc = ln.forecast_score(net, nx, ny, list(range(20000, 20008)))
+    # step cost of the coarse model with and without the correction (8 members)
+    m = dy.DryPrimitiveModel(31, 8, nmembers=8)
+    m.set_state(m.jet_state(list(range(8))))
+    m.run(4)
+    corr = ln.corrector(net, nx, ny)
+    ms = {}
+    for tag, c in (("plain", None), ("corrected", corr)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m.run(50, correction=c)
+        torch.cuda.synchronize()
+        ms[tag] = (time.perf_counter() - t0) / 50 * 1e3
+    m.close()
+    net.close()
+    return {"workload": f"T31 L8 coarse step (8 members) with a 32-128-128-32 column MLP trained on "
+                        f"{seeds} T127 L8 runs x 2 samples ({x.shape[0]} columns), {steps} Adam steps of 4096",
+            "data_gen_s": t_gen, "train_s": t_fit, "train_columns_per_s": steps * 4096 / t_fit,
+            "final_loss": losses[-1], "heldout_mse_ratio": held, "forecast_1h_error_ratio": fc,
+            "step_ms": ms}
+
+
 def cpu_baseline(spectrum_o, grid_o, coeffs, seconds=10.0):
     """The oracle's FP32 C restatement of the reference CPU path on the host cores, timed on
     whole round trips of the same workload until `seconds` have elapsed (at least one)."""
@@ -191,6 +229,8 @@ def main():
     ap.add_argument("--precision", default="fp16x3", choices=["fp16x3", "bf16x3", "fp32"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="headline workload only (for profiling)")
+    ap.add_argument("--c5-seeds", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -279,8 +319,10 @@ def main():
         "max_rel_err_vs_fp64": {p: e["worst"] for p, e in errs.items()},
         "rel_err_per_field_band": {p: e["per_field_band"] for p, e in errs.items()},
     }
-    out["c2_t127_vector_path"] = bench_c2(sw, torch, args.steps, args.warmup)
-    out["c4_uneven_batch64"] = bench_c4(sw, torch, args.steps, args.warmup)
+    if not args.no_extra:
+        out["c2_t127_vector_path"] = bench_c2(sw, torch, args.steps, args.warmup)
+        out["c4_uneven_batch64"] = bench_c4(sw, torch, args.steps, args.warmup)
+        out["c5_learned_correction"] = bench_c5(sw, seeds=args.c5_seeds)
     if not args.no_cpu:
         cb = cpu_baseline(OSpectrum(lmax, mmax), OGrid("octahedral", n), coeffs, args.cpu_seconds)
         out["cpu_baseline"] = cb
