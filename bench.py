@@ -198,6 +198,19 @@ def bench_c5(sw, seeds=40, heldout=8, steps=3000):
             "step_ms": ms}
 
 
+def measured_traffic(kernel, precision):
+    """HBM bytes per launch (fetch + write) of `kernel` from the committed rocprofv3
+    counter summary (FETCH_SIZE / WRITE_SIZE, scaled by a plain copy), or None."""
+    path = os.path.join(REPO, "profiles", "traffic_t255_k64.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 4 and not line.startswith("#") and f[0] == kernel and precision in f[1].split("+"):
+            return int(f[2]) + int(f[3])
+    return None
+
+
 def cpu_baseline(spectrum_o, grid_o, coeffs, seconds=10.0):
     """The oracle's FP32 C restatement of the reference CPU path on the host cores, timed on
     whole round trips of the same workload until `seconds` have elapsed (at least one)."""
@@ -231,6 +244,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="headline workload only (for profiling)")
     ap.add_argument("--c5-seeds", type=int, default=40)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run three plain 256 MiB device copies (scale for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -285,7 +300,8 @@ def main():
     else:
         roof = {"bound": "mfma", "achieved": flops / t / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    roof["traffic"] = measured_traffic(dominant, args.precision)
+    roof["traffic_source"] = "profiles/traffic_t255_k64.txt (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
     roof["kernel"] = dominant
     roof["kernel_ms"] = kms[dominant]
 
@@ -323,6 +339,12 @@ def main():
         out["c2_t127_vector_path"] = bench_c2(sw, torch, args.steps, args.warmup)
         out["c4_uneven_batch64"] = bench_c4(sw, torch, args.steps, args.warmup)
         out["c5_learned_correction"] = bench_c5(sw, seeds=args.c5_seeds)
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
     if not args.no_cpu:
         cb = cpu_baseline(OSpectrum(lmax, mmax), OGrid("octahedral", n), coeffs, args.cpu_seconds)
         out["cpu_baseline"] = cb
