@@ -53,6 +53,7 @@ def kernel_cost(name, K, M, nharm, npoints, nfreq_used, nlon, split):
 def run_mode(sw, torch, precision, spectrum, grid, coeffs_host, steps, warmup):
     K = coeffs_host.shape[0]
     S = sw.SpectralTransform(spectrum, grid, nlayers=K, precision=precision)
+    tuning = S.autotune(K)  # plan-time choice of FFT policy and launch orders (not timed)
     coeffs = sw.LowerTriangularArray(torch.from_numpy(coeffs_host).to(torch.complex64).cuda(), spectrum)
     field = sw.Field(torch.empty((K, grid.npoints), device="cuda"), grid)
     back = sw.LowerTriangularArray(torch.empty((K, spectrum.nharm), dtype=torch.complex64, device="cuda"), spectrum)
@@ -83,7 +84,7 @@ def run_mode(sw, torch, precision, spectrum, grid, coeffs_host, steps, warmup):
     out = back.data.cpu().numpy().astype(np.complex128)
     S.close()
     return {"s_per_step": dt_plain / steps, "s_per_step_profiled": dt / steps,
-            "kernel_ms": {KERNELS[i]: avg[i] for i in range(4)}, "roundtrip": out}
+            "kernel_ms": {KERNELS[i]: avg[i] for i in range(4)}, "roundtrip": out, "tuning": tuning}
 
 
 def time_loop(torch, fn, steps, warmup):
@@ -132,6 +133,7 @@ def bench_c4(sw, torch, steps, warmup):
     sizes = {31: (24, 33, 32), 63: (48, 65, 64), 127: (96, 129, 128), 255: (192, 257, 256)}
     plans = {T: sw.SpectralTransform(sw.Spectrum(l, m), sw.OctahedralGaussianGrid(n), nlayers=64, precision="fp32")
              for T, (n, l, m) in sizes.items()}
+    tuned = {f"T{T}": S.autotune(16) for T, S in plans.items()}  # per size class, plan time
     rng = np.random.default_rng(1234)
     items = []
     for i in range(64):
@@ -153,7 +155,7 @@ def bench_c4(sw, torch, steps, warmup):
     dt_seq = time_loop(torch, sequential, max(steps // 4, 5), max(warmup // 4, 2))
     res = {"batch_ms": dt_batch * 1e3, "transforms_per_s": 64.0 / dt_batch, "sequential_ms": dt_seq * 1e3,
            "speedup_vs_sequential": dt_seq / dt_batch, "uses_graph": B.uses_graph,
-           "total_columns": int(sum(it[1].nlayers for it in items))}
+           "total_columns": int(sum(it[1].nlayers for it in items)), "autotuned": tuned}
     B.close()
     for S in plans.values():
         S.close()
@@ -331,6 +333,7 @@ def main():
         "fp32_round_trips_per_s": 1.0 / res["fp32"]["s_per_step"],
         "speedup_vs_fp32": res["fp32"]["s_per_step"] / head["s_per_step"],
         "kernel_ms": {p: r["kernel_ms"] for p, r in res.items()},
+        "autotuned": {p: r["tuning"] for p, r in res.items()},
         "roundtrip_roofline_frac": t_roof / head["s_per_step"],
         "max_rel_err_vs_fp64": {p: e["worst"] for p, e in errs.items()},
         "rel_err_per_field_band": {p: e["per_field_band"] for p, e in errs.items()},

@@ -162,6 +162,20 @@ int sht_mlp_train_step(sht_mlp *mlp, const float *x, const float *t, int32_t B, 
                        void *stream);
 int sht_mlp_destroy(sht_mlp *mlp);
 
+/* Autotuner (no reference counterpart; tile and order choices per size class): on K
+ * columns of scratch data, times the ring FFT's factorisation policies (first-pass MFMA
+ * threshold 11 or 17, powers of two as 8s or 4s, small odd radices first or last) and the
+ * launch orders (Legendre tiles grouped per order m on one XCD or plain heaviest-first;
+ * FFT synthesis column groups paired or not), keeps the fastest in the plan and caches it
+ * per (grid, nlat_half, lmax, mmax, nlayers, precision, K) for later plans in the process.
+ * Synchronises the stream; results are unchanged up to FP32 rounding of the FFT order. */
+int sht_plan_autotune(sht_plan *plan, int32_t K, void *stream);
+/* out[6] = {tuned, MFMA threshold, eights, odd_first, Legendre XCD grouping, FFT pairing} */
+int sht_plan_tuning(const sht_plan *plan, int32_t *out);
+/* pins a choice: in[5] = {MFMA threshold (11 or 17), eights, odd_first, Legendre XCD
+ * grouping, FFT pairing}; synchronises the device */
+int sht_plan_set_tuning(sht_plan *plan, const int32_t *in);
+
 /* Per-kernel timing (no reference counterpart; the reference profiled ad hoc with
  * CUDA.@profile / BenchmarkTools, docs/dev/2026-08/legendre-gpu-optimization.md).
  * Between begin and end, the first max_calls launches of every kernel are bracketed by

@@ -26,6 +26,7 @@ EXPORTS = (
     "sht_version", "sht_profile_begin", "sht_profile_end", "sht_batch_create", "sht_batch_launch",
     "sht_batch_uses_graph", "sht_batch_destroy", "sht_mlp_nparams", "sht_mlp_create", "sht_mlp_set_params",
     "sht_mlp_get_params", "sht_mlp_forward", "sht_mlp_gradient", "sht_mlp_train_step", "sht_mlp_destroy",
+    "sht_plan_autotune", "sht_plan_tuning", "sht_plan_set_tuning",
 )
 
 
@@ -90,6 +91,9 @@ def lib() -> C.CDLL:
         L.sht_mlp_gradient.argtypes = [vp, vp, vp, i32, vp, vp, vp]
         L.sht_mlp_train_step.argtypes = [vp, vp, vp, i32, C.c_float, vp, vp]
         L.sht_mlp_destroy.argtypes = [vp]
+        L.sht_plan_autotune.argtypes = [vp, i32, vp]
+        L.sht_plan_tuning.argtypes = [vp, vp]
+        L.sht_plan_set_tuning.argtypes = [vp, vp]
         L.sht_profile_begin.argtypes = [vp, i32]
         L.sht_profile_end.argtypes = [vp, vp, vp]
         L.sht_last_error.restype = C.c_char_p

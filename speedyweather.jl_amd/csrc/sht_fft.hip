@@ -473,7 +473,7 @@ template <bool INV>
 __device__ __attribute__((always_inline)) void fft_run(float2 *buf, const float2 *tw, const SHT_GLOBAL int *fac, int nf, int N, int ncol,
                         int tid) {
     int l = 1, f = 0;
-    if (nf > 0 && fac[0] >= kMfmaMinRadix) {
+    if (nf > 0 && fac[0] >= kMfmaMinPrime) {
         fft_pass_mfma_first<INV>(buf, tw, N, fac[0], ncol, tid);
         l = fac[0];
         f = 1;
@@ -524,14 +524,14 @@ struct FFTTile {
     int j, N, mt, k0, ncol;
 };
 
-// Single-call launch orders, rings heaviest first.  PAIRED (synthesis, which reads the
+// Single-call launch orders, rings heaviest first.  Paired (synthesis, which reads the
 // Fourier rows): units (ring rank r, column-group pair p) go round-robin to the XCDs and
 // the two column groups of a unit, which share the 128-byte lines of the Fourier rows, run
 // back to back on the same XCD: block b -> XCD b % 8, slot s = b / 8, unit
 // (s / 2) * 8 + b % 8, column group 2 p + s % 2.  Otherwise (analysis): block b -> ring
 // rank b / kgs, column group b % kgs.  Returns false for blocks past the end.
-template <bool BATCH, bool PAIRED>
-__device__ inline bool fft_tile(const DevPlan &P, const Tile &T, int K, FFTTile &t) {
+template <bool BATCH>
+__device__ inline bool fft_tile(const DevPlan &P, const Tile &T, int K, bool paired, FFTTile &t) {
     const int kgs = (K + kFFTCols - 1) / kFFTCols;
     int cg;
     if (BATCH) {
@@ -541,7 +541,7 @@ __device__ inline bool fft_tile(const DevPlan &P, const Tile &T, int K, FFTTile 
     } else {
         const int b = blockIdx.x;
         int r;
-        if (PAIRED) {
+        if (paired) {
             const int npair = (kgs + 1) >> 1;
             const int s = b / kXCDs;
             const int u = (s >> 1) * kXCDs + b % kXCDs;
@@ -580,7 +580,7 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_synthesis_kernel(const Dev
     const DevPlan &P = plans[C.plan];
     const int K = C.K;
     FFTTile G;
-    if (!fft_tile<BATCH, true>(P, T, K, G)) return;  // padding of the launch order
+    if (!fft_tile<BATCH>(P, T, K, C.paired != 0, G)) return;  // padding of the launch order
     const int N = G.N, mt = G.mt, ncol = G.ncol;
     float2 *tw = smem;
     float2 *b0 = smem + N;
@@ -656,7 +656,7 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_analysis_kernel(const DevP
     const DevPlan &P = plans[C.plan];
     const int K = C.K;
     FFTTile G;
-    if (!fft_tile<BATCH, false>(P, T, K, G)) return;  // padding of the launch order
+    if (!fft_tile<BATCH>(P, T, K, false, G)) return;  // padding of the launch order
     const int N = G.N, mt = G.mt, ncol = G.ncol;
     float2 *tw = smem;
     float2 *b0 = smem + N;

@@ -21,8 +21,11 @@ constexpr int kFFTThreads = 256;
 // Largest ring the FFT kernel takes, and columns per FFT workgroup (one wave each).
 constexpr int kFFTMaxRing = 1024;
 constexpr int kFFTCols = 4;
-// First-pass prime radices from this size up run on MFMA; smaller radices (and any
-// later pass) run as register butterflies, which exist for 2, 3, 4, 5, 7, 11, 13, 17, 19.
+// A first radix of at least kMfmaMinPrime runs as the MFMA pass; the factorisation puts
+// every prime from kMfmaMinRadix up first (the largest one), and 11 or 13 first only when
+// the tuned policy asks for it.  Later passes run as register butterflies, which exist for
+// 2, 3, 4, 5, 7, 8, 11, 13, 17, 19.
+constexpr int kMfmaMinPrime = 11;
 constexpr int kMfmaMinRadix = 17;
 constexpr int kMaxRegisterRadix = 19;
 // MFMA GEMM pass of the FFT: tiles (16 output frequencies x 16 real columns) a wave keeps
@@ -53,7 +56,7 @@ struct CallArgs {
     int plan;             // index into the launch's plan array
     int K;                // columns
     int unscale;          // synthesis: divide by cos(lat)
-    int pad;
+    int paired;           // single-call FFT synthesis: paired column-group launch order
     const void *in;       // synthesis: coefficients; analysis: grid
     void *out;            // synthesis: grid; analysis: coefficients
     long long ld_in;      // column strides (complex elements for spectra, floats for grids)

@@ -12,7 +12,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numeric>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -219,6 +222,13 @@ struct Profile {
     std::vector<hipEvent_t> ev;  // [(kernel * max_calls + call) * 2 + {start, stop}]
 };
 
+// Ring FFT factorisation policy (a tuning knob of the autotuner).
+struct FFTPolicy {
+    int mfma_min = 17;       // a largest prime >= this runs first, as the MFMA pass
+    bool eights = true;      // powers of two as 8s (then one 4 or 2) rather than 4s
+    bool odd_first = false;  // small odd radices before the powers of two
+};
+
 // A launch order for workgroups grouped in units that share data (tiles of one unit run
 // back to back on one XCD): units, heaviest first, go greedily to the least-loaded XCD;
 // block b then takes slot b / 8 of XCD b % 8's list.  Lists are padded to equal length
@@ -273,9 +283,15 @@ struct sht_plan {
     DevPlan P{};
     const DevPlan *dP = nullptr;  // device copy of P
     std::vector<void *> allocs;
-    // single-call Legendre launch orders, one per column-group count cgs = 1..max
-    std::vector<const Tile *> syn_tiles, ana_tiles;
-    std::vector<int> nsyn, nana;
+    // single-call Legendre launch orders, one per column-group count cgs = 1..max, grouped
+    // by order m per XCD ([1]) or plain heavy-first ([0])
+    std::vector<const Tile *> syn_tiles[2], ana_tiles[2];
+    std::vector<int> nsyn[2], nana[2];
+    // tuned choices (sht_plan_autotune): FFT factorisation, Legendre order grouping, FFT
+    // synthesis column-group pairing
+    FFTPolicy fft_policy;
+    int leg_xcd = 1, fft_pair = 1;
+    int tuned = 0;
     // host copies for batches: Legendre tiles (heavy first) with their costs, FFT ring order
     std::vector<Tile> h_syn, h_ana;
     std::vector<double> h_syn_cost, h_ana_cost, h_ring_cost;
@@ -331,37 +347,79 @@ void free_plan(sht_plan *p) {
     delete p;
 }
 
-// Radices of the ring FFT, in pass order: the largest prime >= kMfmaMinRadix first (the
-// first pass runs as an MFMA GEMM and needs no twiddles), then 8s, 4, 2, 3, 5, 7, 11, 13,
-// then any further prime >= 17.  Returns the count, or -1 if a radix has no kernel.
-int factorize(int N, int *fac) {
+// Radices of the ring FFT, in pass order: the largest prime >= policy.mfma_min first (the
+// first pass runs as an MFMA GEMM and needs no twiddles), then the powers of two (8s or
+// 4s) and 2, 3, 5, 7, 11, 13 (odd ones first or last), then any further prime >= 11.
+// Returns the count, or -1 if a radix has no kernel.
+int factorize(int N, int *fac, const FFTPolicy &pol) {
     int nf = 0, r = N;
-    std::vector<int> big;
+    std::vector<int> big;  // primes >= 11, largest first
     int tmp = r;
-    for (int q : {2, 3, 5, 7, 11, 13})
+    for (int q : {2, 3, 5, 7})
         while (tmp % q == 0) tmp /= q;
-    for (int p = 17; p <= tmp; p += 2) {  // tmp has no factor below 17: p divides only if prime
+    for (int p = 11; p <= tmp; p += 2) {  // tmp has no factor below 11: p divides only if prime
         while (tmp % p == 0) {
             big.push_back(p);
             tmp /= p;
         }
     }
     std::sort(big.begin(), big.end(), [](int a, int b) { return a > b; });
-    if (!big.empty()) {
+    size_t used = 0;
+    if (!big.empty() && big[0] >= std::min(pol.mfma_min, sht::kMfmaMinRadix)) {
         if (big[0] > sht::kMaxMfmaRadix) return -1;
         fac[nf++] = big[0];
         r /= big[0];
+        used = 1;
     }
-    while (r % 8 == 0) { fac[nf++] = 8; r /= 8; }
-    if (r % 4 == 0) { fac[nf++] = 4; r /= 4; }
-    for (int q : {2, 3, 5, 7, 11, 13})
-        while (r % q == 0) { fac[nf++] = q; r /= q; }
-    for (size_t i = 1; i < big.size(); ++i) {
+    auto pow2 = [&]() {
+        if (pol.eights) {
+            while (r % 8 == 0) { fac[nf++] = 8; r /= 8; }
+        }
+        while (r % 4 == 0) { fac[nf++] = 4; r /= 4; }
+        while (r % 2 == 0) { fac[nf++] = 2; r /= 2; }
+    };
+    auto odd = [&]() {
+        for (int q : {3, 5, 7})
+            while (r % q == 0) { fac[nf++] = q; r /= q; }
+    };
+    if (pol.odd_first) {
+        odd();
+        pow2();
+    } else {
+        pow2();
+        odd();
+    }
+    for (size_t i = used; i < big.size(); ++i) {
         if (big[i] > sht::kMaxRegisterRadix) return -1;
         fac[nf++] = big[i];
         r /= big[i];
     }
     return r == 1 ? nf : -1;
+}
+
+// FFT tables of a policy: radices per ring, modelled cost and heavy-first ring order.
+int fft_tables(const Geometry &g, const FFTPolicy &pol, std::vector<int> &fac, std::vector<int> &nfac,
+               std::vector<double> &cost, std::vector<int> &order) {
+    const int n = g.n;
+    fac.assign((size_t)n * sht::kMaxFactors, 1);
+    nfac.assign(n, 0);
+    cost.assign(n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        nfac[j] = factorize(g.nlon[j], &fac[(size_t)j * sht::kMaxFactors], pol);
+        if (nfac[j] < 0)
+            return fail(SHT_ERR_UNSUPPORTED,
+                        "ring length " + std::to_string(g.nlon[j]) + " has a prime factor the FFT cannot run");
+        double c = 0;
+        for (int f = 0; f < nfac[j]; ++f) {
+            const int r = fac[(size_t)j * sht::kMaxFactors + f];
+            c += r <= 8 ? 1.0 : (f == 0 ? 1.0 + r / 48.0 : 0.5 * r);
+        }
+        cost[j] = c * g.nlon[j];
+    }
+    order.resize(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    return SHT_OK;
 }
 
 int build_plan(sht_plan *pl) {
@@ -498,13 +556,12 @@ int build_plan(sht_plan *pl) {
     }
 
     // ------------------------------------------------------------- FFT tables
-    std::vector<int> fac((size_t)n * sht::kMaxFactors, 1), nfac(n);
+    std::vector<int> fac, nfac, ring_order;
+    std::vector<double> ring_cost;
+    if ((rc = fft_tables(g, pl->fft_policy, fac, nfac, ring_cost, ring_order))) return rc;
     std::vector<long long> tw_off(n);
     long long tw_total = 0;
     for (int j = 0; j < n; ++j) {
-        nfac[j] = factorize(g.nlon[j], &fac[(size_t)j * sht::kMaxFactors]);
-        if (nfac[j] < 0)
-            return fail(SHT_ERR_UNSUPPORTED, "ring length " + std::to_string(g.nlon[j]) + " has a prime factor the FFT cannot run");
         tw_off[j] = tw_total;
         tw_total += g.nlon[j];
     }
@@ -517,18 +574,6 @@ int build_plan(sht_plan *pl) {
             tw[2 * (tw_off[j] + k) + 1] = (float)std::sin(a);
         }
     }
-    std::vector<double> ring_cost(n);
-    for (int j = 0; j < n; ++j) {
-        double c = 0;
-        for (int f = 0; f < nfac[j]; ++f) {
-            const int r = fac[(size_t)j * sht::kMaxFactors + f];
-            c += r <= 8 ? 1.0 : (r < sht::kMfmaMinRadix ? 0.5 * r : 0.25 * r);
-        }
-        ring_cost[j] = c * g.nlon[j];
-    }
-    std::vector<int> ring_order(n);
-    std::iota(ring_order.begin(), ring_order.end(), 0);
-    std::stable_sort(ring_order.begin(), ring_order.end(), [&](int a, int b) { return ring_cost[a] > ring_cost[b]; });
     if (g.nlon_max > sht::kFFTMaxRing) return fail(SHT_ERR_UNSUPPORTED, "ring too long for the LDS FFT");
     const int kg = sht::kFFTCols;
     P.kg = kg;
@@ -625,17 +670,33 @@ int build_plan(sht_plan *pl) {
     if ((rc = dev_upload(A, l_of, &P.l_of))) return rc;
     {
         const int cgs_max = (d.nlayers + sht::kColsComplex - 1) / sht::kColsComplex;
-        for (int cgs = 1; cgs <= cgs_max; ++cgs) {
-            const std::vector<Tile> so = xcd_order(legendre_units(syn_t, syn_c, 0, cgs));
-            const std::vector<Tile> ao = xcd_order(legendre_units(ana_t, ana_c, 0, cgs));
-            const Tile *ds = nullptr, *da = nullptr;
-            if ((rc = dev_upload(A, so, &ds))) return rc;
-            if ((rc = dev_upload(A, ao, &da))) return rc;
-            pl->syn_tiles.push_back(ds);
-            pl->ana_tiles.push_back(da);
-            pl->nsyn.push_back((int)so.size());
-            pl->nana.push_back((int)ao.size());
-        }
+        for (int cgs = 1; cgs <= cgs_max; ++cgs)
+            for (int grouped = 0; grouped < 2; ++grouped) {
+                std::vector<LaunchUnit> su = legendre_units(syn_t, syn_c, 0, cgs);
+                std::vector<LaunchUnit> au = legendre_units(ana_t, ana_c, 0, cgs);
+                std::vector<Tile> so, ao;
+                if (grouped) {
+                    so = xcd_order(su);
+                    ao = xcd_order(au);
+                } else {
+                    // tiles heaviest first, one after another
+                    for (auto *uo : {&su, &au}) {
+                        std::vector<std::pair<double, Tile>> t;
+                        for (const LaunchUnit &u : *uo)
+                            for (const Tile &x : u.tiles) t.push_back({u.cost / u.tiles.size(), x});
+                        std::stable_sort(t.begin(), t.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+                        std::vector<Tile> &o = uo == &su ? so : ao;
+                        for (auto &x : t) o.push_back(x.second);
+                    }
+                }
+                const Tile *ds = nullptr, *da = nullptr;
+                if ((rc = dev_upload(A, so, &ds))) return rc;
+                if ((rc = dev_upload(A, ao, &da))) return rc;
+                pl->syn_tiles[grouped].push_back(ds);
+                pl->ana_tiles[grouped].push_back(da);
+                pl->nsyn[grouped].push_back((int)so.size());
+                pl->nana[grouped].push_back((int)ao.size());
+            }
     }
     {
         std::vector<DevPlan> one(1, P);
@@ -760,17 +821,18 @@ int sht_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *fie
     if (ld_grid < plan->g.npoints) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_grid < npoints");
     DeviceGuard guard(plan->d.device);
     const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
-    const CallArgs c{0, K, unscale_coslat ? 1 : 0, 0, coeffs, field, ld_spec, ld_grid, plan->fbuf};
+    const CallArgs c{0, K, unscale_coslat ? 1 : 0, plan->fft_pair, coeffs, field, ld_spec, ld_grid, plan->fbuf};
     hipError_t e;
     {
         ProfScope ps(plan, SHT_KERNEL_LEGENDRE_SYNTHESIS, stream);
+        const int gx = plan->leg_xcd;
         e = launch_stage(sht::legendre_kernel(sht::kLegendreSynthesis, plan->d.precision, false),
-                         dim3(plan->nsyn[cgs - 1]), 0, plan->dP, c, nullptr, plan->syn_tiles[cgs - 1], stream);
+                         dim3(plan->nsyn[gx][cgs - 1]), 0, plan->dP, c, nullptr, plan->syn_tiles[gx][cgs - 1], stream);
     }
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("legendre synthesis launch: ") + hipGetErrorString(e));
     {
         ProfScope ps(plan, SHT_KERNEL_FFT_SYNTHESIS, stream);
-        e = launch_stage(sht::fft_kernel(sht::kFFTSynthesis, false), dim3(fft_blocks(plan->g.n, K, true)),
+        e = launch_stage(sht::fft_kernel(sht::kFFTSynthesis, false), dim3(fft_blocks(plan->g.n, K, plan->fft_pair != 0)),
                          plan->P.fft_smem, plan->dP, c, nullptr, nullptr, stream);
     }
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("fft synthesis launch: ") + hipGetErrorString(e));
@@ -795,8 +857,9 @@ int sht_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void *coeff
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("fft analysis launch: ") + hipGetErrorString(e));
     {
         ProfScope ps(plan, SHT_KERNEL_LEGENDRE_ANALYSIS, stream);
+        const int gx = plan->leg_xcd;
         e = launch_stage(sht::legendre_kernel(sht::kLegendreAnalysis, plan->d.precision, false),
-                         dim3(plan->nana[cgs - 1]), 0, plan->dP, c, nullptr, plan->ana_tiles[cgs - 1], stream);
+                         dim3(plan->nana[gx][cgs - 1]), 0, plan->dP, c, nullptr, plan->ana_tiles[gx][cgs - 1], stream);
     }
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("legendre analysis launch: ") + hipGetErrorString(e));
     return SHT_OK;
@@ -851,6 +914,162 @@ int sht_profile_end(sht_plan *plan, double *avg_ms, int32_t *counts) {
     for (hipEvent_t e : pr.ev) (void)hipEventDestroy(e);
     pr = Profile();
     return rc;
+}
+
+namespace {
+// Uploads the FFT tables of a factorisation policy into the plan's device arrays.
+int apply_fft_policy(sht_plan *plan, const FFTPolicy &pol, void *stream) {
+    std::vector<int> fac, nfac, order;
+    std::vector<double> cost;
+    int rc = fft_tables(plan->g, pol, fac, nfac, cost, order);
+    if (rc) return rc;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (hipStreamSynchronize(st) != hipSuccess ||
+        hipMemcpy(const_cast<int *>(plan->P.fac), fac.data(), fac.size() * sizeof(int), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(const_cast<int *>(plan->P.nfac), nfac.data(), nfac.size() * sizeof(int), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(const_cast<int *>(plan->P.ring_order), order.data(), order.size() * sizeof(int),
+                  hipMemcpyHostToDevice) != hipSuccess)
+        return fail(SHT_ERR_HIP, "uploading FFT tables failed");
+    plan->fft_policy = pol;
+    plan->h_ring_order = order;
+    plan->h_ring_cost = cost;
+    return SHT_OK;
+}
+
+struct TuneKey {
+    int grid, nlat_half, lmax, mmax, nlayers, precision, K;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(grid, nlat_half, lmax, mmax, nlayers, precision, K) <
+               std::tie(o.grid, o.nlat_half, o.lmax, o.mmax, o.nlayers, o.precision, o.K);
+    }
+};
+struct TuneChoice {
+    FFTPolicy pol;
+    int leg_xcd, fft_pair;
+};
+std::map<TuneKey, TuneChoice> g_tune_cache;
+std::mutex g_tune_mutex;
+}  // namespace
+
+int sht_plan_autotune(sht_plan *plan, int32_t K, void *stream) {
+    if (!plan) return fail(SHT_ERR_ARG, "null plan");
+    if (K < 1 || K > plan->d.nlayers) return fail(SHT_ERR_DIM, "DimensionMismatch: K outside 1..nlayers");
+    DeviceGuard guard(plan->d.device);
+    const TuneKey key{plan->d.grid, plan->d.nlat_half, plan->d.lmax, plan->d.mmax, plan->d.nlayers,
+                      plan->d.precision, K};
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            const int rc = apply_fft_policy(plan, it->second.pol, stream);
+            if (rc) return rc;
+            plan->leg_xcd = it->second.leg_xcd;
+            plan->fft_pair = it->second.fft_pair;
+            plan->tuned = 1;
+            return SHT_OK;
+        }
+    }
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    void *coeffs = nullptr, *field = nullptr;
+    const size_t cb = (size_t)plan->g.nharm * K * 8, fb = (size_t)plan->g.npoints * K * 4;
+    if (hipMalloc(&coeffs, cb) != hipSuccess || hipMalloc(&field, fb) != hipSuccess) {
+        if (coeffs) (void)hipFree(coeffs);
+        return fail(SHT_ERR_ALLOC, "autotune scratch allocation failed");
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SHT_OK;
+    if (hipMemsetAsync(coeffs, 0, cb, st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess)
+        rc = fail(SHT_ERR_HIP, "autotune set-up failed");
+    // mean round-trip time over `reps` calls after two untimed ones
+    auto round_trip_ms = [&](float &ms) -> int {
+        const int reps = 6;
+        for (int i = 0; i < reps + 2; ++i) {
+            if (i == 2) (void)hipEventRecord(e0, st);
+            int r = sht_synthesis(plan, coeffs, plan->g.nharm, field, plan->g.npoints, K, 0, stream);
+            if (!r) r = sht_analysis(plan, field, plan->g.npoints, coeffs, plan->g.nharm, K, stream);
+            if (r) return r;
+        }
+        (void)hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            return fail(SHT_ERR_HIP, "autotune timing failed");
+        ms /= reps;
+        return SHT_OK;
+    };
+    TuneChoice best{plan->fft_policy, plan->leg_xcd, plan->fft_pair};
+    if (!rc) {
+        float best_ms = 1e30f;
+        for (int mmin : {17, 11})
+            for (int eights = 1; eights >= 0; --eights)
+                for (int odd_first = 0; odd_first < 2 && !rc; ++odd_first) {
+                    FFTPolicy pol;
+                    pol.mfma_min = mmin;
+                    pol.eights = eights != 0;
+                    pol.odd_first = odd_first != 0;
+                    float ms = 0.f;
+                    if ((rc = apply_fft_policy(plan, pol, stream)) || (rc = round_trip_ms(ms))) break;
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best.pol = pol;
+                    }
+                }
+        if (!rc) rc = apply_fft_policy(plan, best.pol, stream);
+        best_ms = 1e30f;
+        for (int xcd = 1; xcd >= 0 && !rc; --xcd)
+            for (int pair = 1; pair >= 0 && !rc; --pair) {
+                plan->leg_xcd = xcd;
+                plan->fft_pair = pair;
+                float ms = 0.f;
+                if ((rc = round_trip_ms(ms))) break;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best.leg_xcd = xcd;
+                    best.fft_pair = pair;
+                }
+            }
+        plan->leg_xcd = best.leg_xcd;
+        plan->fft_pair = best.fft_pair;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(coeffs);
+    (void)hipFree(field);
+    if (rc) return rc;
+    plan->tuned = 1;
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tune_cache[key] = best;
+    return SHT_OK;
+}
+
+int sht_plan_set_tuning(sht_plan *plan, const int32_t *in) {
+    if (!plan || !in) return fail(SHT_ERR_ARG, "bad arguments");
+    if ((in[0] != 11 && in[0] != 17) || in[1] < 0 || in[1] > 1 || in[2] < 0 || in[2] > 1 || in[3] < 0 ||
+        in[3] > 1 || in[4] < 0 || in[4] > 1)
+        return fail(SHT_ERR_ARG, "tuning values out of range");
+    DeviceGuard guard(plan->d.device);
+    FFTPolicy pol;
+    pol.mfma_min = in[0];
+    pol.eights = in[1] != 0;
+    pol.odd_first = in[2] != 0;
+    const int rc = apply_fft_policy(plan, pol, nullptr);
+    if (rc) return rc;
+    plan->leg_xcd = in[3];
+    plan->fft_pair = in[4];
+    return SHT_OK;
+}
+
+int sht_plan_tuning(const sht_plan *plan, int32_t *out) {
+    if (!plan || !out) return fail(SHT_ERR_ARG, "bad arguments");
+    out[0] = plan->tuned;
+    out[1] = plan->fft_policy.mfma_min;
+    out[2] = plan->fft_policy.eights ? 1 : 0;
+    out[3] = plan->fft_policy.odd_first ? 1 : 0;
+    out[4] = plan->leg_xcd;
+    out[5] = plan->fft_pair;
+    return SHT_OK;
 }
 
 int sht_gauss_legendre(int32_t nlat, double *nodes, double *weights) {

@@ -59,6 +59,25 @@ class SpectralTransform:
         nlat_half = nlat_half if nlat_half > 0 else get_nlat_half(spectrum.mmax, dealiasing)
         return cls(spectrum, Grid(nlat_half), **kw)
 
+    def autotune(self, K: int | None = None) -> dict:
+        """Times the plan's FFT factorisation policies and launch orders on K columns and
+        keeps the fastest (sht_plan_autotune); returns the choice."""
+        K = self.nlayers if K is None else int(K)
+        check(lib().sht_plan_autotune(self._plan, K, self._stream()), "sht_plan_autotune")
+        return self.tuning()
+
+    def set_tuning(self, mfma_min=17, eights=True, odd_first=False, legendre_xcd_groups=True, fft_pairs=True):
+        """Pins the FFT factorisation policy and launch orders (sht_plan_set_tuning)."""
+        v = (C.c_int32 * 5)(int(mfma_min), int(bool(eights)), int(bool(odd_first)), int(bool(legendre_xcd_groups)),
+                            int(bool(fft_pairs)))
+        check(lib().sht_plan_set_tuning(self._plan, v), "sht_plan_set_tuning")
+
+    def tuning(self) -> dict:
+        out = (C.c_int32 * 6)()
+        check(lib().sht_plan_tuning(self._plan, out), "sht_plan_tuning")
+        return {"tuned": bool(out[0]), "mfma_min": out[1], "eights": bool(out[2]), "odd_first": bool(out[3]),
+                "legendre_xcd_groups": bool(out[4]), "fft_pairs": bool(out[5])}
+
     def close(self):
         if getattr(self, "_plan", None) and self._plan.value:
             lib().sht_plan_destroy(self._plan)
