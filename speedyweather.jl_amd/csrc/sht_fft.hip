@@ -321,60 +321,59 @@ __device__ inline void mfma_prime_rounds(float2 *buf, const float2 *tw, int N, i
                 acc.t2[m][n] = f4{0.f, 0.f, 0.f, 0.f};
             }
         }
-        // operands of K-steps ks (slot 0) and ks + 1 (slot 1)
-        float2 A0[MT], A1[MT];
-        float X0[NT], Y0[NT], X1[NT], Y1[NT];
-        auto fetch = [&](float2 *A, float *X, float *Y) {
+        // Operands of even K-steps in set 0, odd ones in set 1; a set is refilled with the
+        // step two ahead right after its MFMAs, so loads are always two steps in flight.
+        // Fetches past the last step re-read the last step's operands (never used): the
+        // loads stay unconditional and the wait for a set never waits for the other one.
+        struct Ops {
+            float2 A[MT];
+            float X[NT], Y[NT];
+        } S0, S1;
+        int kf = 0;  // next step to fetch
+        auto fetch = [&](Ops &S) {
 #pragma unroll
-            for (int m = 0; m < MT; ++m) {
-                A[m] = tw[e[m]];
-                e[m] += de[m];
-                e[m] -= e[m] >= N ? N : 0;
-            }
+            for (int m = 0; m < MT; ++m) S.A[m] = tw[e[m]];
 #pragma unroll
             for (int n = 0; n < NT; ++n) {
-                X[n] = bf[pa[n]];
-                Y[n] = bf[pb[n]];
-                pa[n] += 8 * M;
-                pb[n] -= 8 * M;
+                S.X[n] = bf[pa[n]];
+                S.Y[n] = bf[pb[n]];
+            }
+            const bool adv = ++kf < Kt;  // uniform
+            if (adv) {
+#pragma unroll
+                for (int m = 0; m < MT; ++m) {
+                    e[m] += de[m];
+                    e[m] -= e[m] >= N ? N : 0;
+                }
+#pragma unroll
+                for (int n = 0; n < NT; ++n) {
+                    pa[n] += 8 * M;
+                    pb[n] -= 8 * M;
+                }
             }
         };
-        fetch(A0, X0, Y0);
-        if (Kt > 1) fetch(A1, X1, Y1);
-        for (int ks = 0; ks < Kt; ++ks) {
-            float2 A2[MT];
-            float X2[NT], Y2[NT];
-#pragma unroll
-            for (int m = 0; m < MT; ++m) A2[m] = make_float2(0.f, 0.f);
-#pragma unroll
-            for (int n = 0; n < NT; ++n) X2[n] = Y2[n] = 0.f;
-            if (ks + 2 < Kt) fetch(A2, X2, Y2);
-            if (ks == Kt - 1 && !last_ok) {
-#pragma unroll
-                for (int m = 0; m < MT; ++m) A0[m] = make_float2(0.f, 0.f);
-            }
+        auto mma = [&](const Ops &S, int ks) {
+            const bool mask = ks == Kt - 1 && !last_ok;  // this lane's t is past h
 #pragma unroll
             for (int m = 0; m < MT; ++m) {
                 if (m >= nmt) break;  // uniform per wave
                 // the table holds w^k = cos - i sin (forward twiddles)
-                const float cs = A0[m].x, sn = -A0[m].y;
+                const float cs = mask ? 0.f : S.A[m].x, sn = mask ? 0.f : -S.A[m].y;
 #pragma unroll
                 for (int n = 0; n < NT; ++n) {
-                    acc.t1[m][n] = mfma_f32(cs, X0[n], acc.t1[m][n]);
-                    acc.t2[m][n] = mfma_f32(sn, Y0[n], acc.t2[m][n]);
+                    acc.t1[m][n] = mfma_f32(cs, S.X[n], acc.t1[m][n]);
+                    acc.t2[m][n] = mfma_f32(sn, S.Y[n], acc.t2[m][n]);
                 }
             }
-#pragma unroll
-            for (int m = 0; m < MT; ++m) {
-                A0[m] = A1[m];
-                A1[m] = A2[m];
-            }
-#pragma unroll
-            for (int n = 0; n < NT; ++n) {
-                X0[n] = X1[n];
-                Y0[n] = Y1[n];
-                X1[n] = X2[n];
-                Y1[n] = Y2[n];
+        };
+        fetch(S0);
+        fetch(S1);
+        for (int ks = 0; ks < Kt; ks += 2) {
+            mma(S0, ks);
+            fetch(S0);
+            if (ks + 1 < Kt) {
+                mma(S1, ks + 1);
+                fetch(S1);
             }
         }
         // x_0 of each column's butterfly, read before any output of the round is written
@@ -443,7 +442,8 @@ __device__ void fft_pass_mfma_first(float2 *buf, const float2 *tw, int N, int r,
     }
     __syncthreads();
     constexpr int NW = kFFTThreads / 64;
-    const int lane = tid & 63, wave = tid >> 6;
+    // the wave index is uniform per wave: keep it (and all tile ownership) scalar
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int Mt = (h + 16) >> 4;         // row tiles (rows 0..h), at most 2 NW
     const int Nt = (2 * nseq + 15) >> 4;  // column tiles over (butterfly, re/im)
     if (Mt > NW) {
