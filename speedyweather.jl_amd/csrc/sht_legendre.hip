@@ -130,7 +130,8 @@ template <int MODE, bool BATCH>
 __global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
                                                                  const CallArgs *__restrict__ calls,
                                                                  const Tile *__restrict__ tiles) {
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    // the wave index is uniform per wave: keep it (and all tile ownership) scalar
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const Tile T = tiles[blockIdx.x];
     if (T.m < 0) return;  // padding of the launch order
     const CallArgs C = BATCH ? calls[T.call] : single;
@@ -165,72 +166,79 @@ __global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) acc[p][cb] = f4{0.f, 0.f, 0.f, 0.f};
 
-    // register prefetch: 8 degrees of one column, and this wave's Λ fragments
-    float2 v[8];
-    auto load_coeffs = [&](int c) {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int o = 64 * c + 8 * g + e;
-            v[e] = make_float2(0.f, 0.f);
-            if (kok && o < L) v[e] = gload(csrc + o);
-        }
+    // Operands of one chunk: 8 degrees of one column (staging role) and this wave's Λ
+    // fragments.  Two sets alternate (no register copies between chunks); the next chunk's
+    // set is loaded while the current chunk's products run.  All loads are unconditional
+    // (degrees past L re-read degree L - 1, absent columns column 0, idle waves their last
+    // ring tile, the step after the last chunk the last chunk again; the staging zeroes what
+    // is not data), so each wait covers exactly the loads it needs.
+    struct Ops {
+        float2 v[8];
+        float af[2][8];
+        V8of<MODE> ah[2], al[2];
     };
-    float af[2][8], afn[2][8];
-    V8of<MODE> ah[2], al[2], ahn[2], aln[2];
-    auto load_lambda = [&](int c, float (&f)[2][8], V8of<MODE> (&h)[2], V8of<MODE> (&lo)[2]) {
-        if (!active) return;
+    const int ltile = active ? tile : Jt - 1;
+    const long long so = as_global(P.syn_off)[m];
+    const SHT_GLOBAL float *Af[2];
+    const SHT_GLOBAL V8of<MODE> *Ah[2], *Al[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
         if constexpr (MODE == kFP32) {
-            const SHT_GLOBAL float *A = as_global(P.syn_f32) + as_global(P.syn_off)[m];
-#pragma unroll
-            for (int p = 0; p < 2; ++p) {
-                const SHT_GLOBAL float *Ap = A + ((long long)(p * Jt + tile) * (nch * kChunk) + c * kChunk) * 16;
-#pragma unroll
-                for (int ks = 0; ks < kChunk / 4; ++ks) f[p][ks] = Ap[(ks * 4 + grp) * 16 + row];
-            }
+            Af[p] = as_global(P.syn_f32) + so + (long long)(p * Jt + ltile) * (nch * kChunk) * 16 + grp * 16 + row;
         } else {
-            const long long so = as_global(P.syn_off)[m];
-            const SHT_GLOBAL V8of<MODE> *Ah =
-                as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.syn_hi) + so));
-            const SHT_GLOBAL V8of<MODE> *Al =
-                as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.syn_lo) + so));
+            const long long fi = (long long)(p * Jt + ltile) * nch * 64 + lane;
+            Ah[p] = as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.syn_hi) + so)) + fi;
+            Al[p] = as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.syn_lo) + so)) + fi;
+        }
+    }
+    auto load = [&](Ops &S, int c) {
+        c = min(c, nch - 1);
 #pragma unroll
-            for (int p = 0; p < 2; ++p) {
-                const long long fi = ((long long)(p * Jt + tile) * nch + c) * 64 + lane;
-                h[p] = Ah[fi];
-                lo[p] = Al[fi];
+        for (int e = 0; e < 8; ++e) S.v[e] = gload(csrc + min(64 * c + 8 * g + e, L - 1));
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            if constexpr (MODE == kFP32) {
+#pragma unroll
+                for (int ks = 0; ks < kChunk / 4; ++ks) S.af[p][ks] = Af[p][c * kChunk * 16 + ks * 64];
+            } else {
+                S.ah[p] = Ah[p][c * 64];
+                S.al[p] = Al[p][c * 64];
             }
         }
     };
-
-    load_coeffs(0);
-    load_lambda(0, afn, ahn, aln);
-    for (int c = 0; c < nch; ++c) {
+    // stage chunk c from `cur`, load chunk c + 1 into `nxt`, then the products of chunk c
+    auto step = [&](Ops &cur, Ops &nxt, int c) {
         const int buf = c & 1;
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
             float re[4], im[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                re[i] = v[2 * i + p].x;
-                im[i] = v[2 * i + p].y;
+                const bool ok = kok && 64 * c + 8 * g + 2 * i + p < L;
+                re[i] = ok ? cur.v[2 * i + p].x : 0.f;
+                im[i] = ok ? cur.v[2 * i + p].y : 0.f;
             }
             stage4<MODE>(sm, buf, p, kk, g, re, im);
         }
-        if (c + 1 < nch) load_coeffs(c + 1);
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-#pragma unroll
-            for (int ks = 0; ks < 8; ++ks) af[p][ks] = afn[p][ks];
-            ah[p] = ahn[p];
-            al[p] = aln[p];
-        }
-        if (c + 1 < nch) load_lambda(c + 1, afn, ahn, aln);
+        load(nxt, c + 1);
         __syncthreads();
         if (active) {
 #pragma unroll
-            for (int p = 0; p < 2; ++p) mma_chunk<MODE, 4>(sm, buf, p, row, grp, af[p], ah[p], al[p], acc[p]);
+            for (int p = 0; p < 2; ++p)
+                mma_chunk<MODE, 4>(sm, buf, p, row, grp, cur.af[p], cur.ah[p], cur.al[p], acc[p]);
         }
+    };
+
+    // straight-line loop body (a conditional second step would make the waits at the loop
+    // head cover every outstanding load)
+    Ops S0, S1;
+    load(S0, 0);
+    int c = 0;
+    for (; c + 1 < nch; c += 2) {
+        step(S0, S1, c);
+        step(S1, S0, c + 1);
     }
+    if (c < nch) step(S0, S1, c);
 
     if (!active) return;  // no further barriers below
     // epilogue: N = E + O, S = E - O (legendre.jl:84-91), optional 1/cos(lat)
@@ -282,7 +290,8 @@ template <int MODE, bool BATCH>
 __global__ __launch_bounds__(256) void legendre_analysis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
                                                                 const CallArgs *__restrict__ calls,
                                                                 const Tile *__restrict__ tiles) {
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    // the wave index is uniform per wave: keep it (and all tile ownership) scalar
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const Tile T = tiles[blockIdx.x];
     if (T.m < 0) return;  // padding of the launch order
     const CallArgs C = BATCH ? calls[T.call] : single;
@@ -316,64 +325,80 @@ __global__ __launch_bounds__(256) void legendre_analysis_kernel(const DevPlan *_
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) acc[cb] = f4{0.f, 0.f, 0.f, 0.f};
 
-    float4 v[4];
-    auto load_fourier = [&](int c) {
+    // Operands of one chunk: 4 rings of one column (staging role) and this wave's Λ
+    // fragment.  Two sets alternate (no register copies between chunks); the next chunk's
+    // set is loaded while the current chunk's products run.  All loads are unconditional
+    // (rings past n re-read ring n - 1, absent columns column 0, the step after the last
+    // chunk the last chunk again; the staging zeroes what is not data), so each wait covers
+    // exactly the loads it needs.  A set also carries the Fourier row offsets of the chunk
+    // after its own, loaded ahead of its rows, so no Fourier load waits for the offset table.
+    struct Ops {
+        float4 v[4];
+        float af[8];
+        V8of<MODE> ah, al;
+        long long fo[4];  // ring offsets in the Fourier buffer of the following chunk
+    };
+    const long long ao = as_global(P.ana_off)[m];
+    const SHT_GLOBAL float *Af =
+        as_global(P.ana_f32) + ao + (long long)(p * nt + lt) * (JC * kChunk) * 16 + grp * 16 + row;
+    const long long fi = (long long)(p * nt + lt) * JC * 64 + lane;
+    const SHT_GLOBAL V8of<MODE> *Ah =
+        as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_hi) + ao)) + fi;
+    const SHT_GLOBAL V8of<MODE> *Al =
+        as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_lo) + ao)) + fi;
+    auto load_offsets = [&](long long (&fo)[4], int c) {
+        c = min(c, JC - 1);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) fo[i] = as_global(P.fboff)[min(j0 + kChunk * c + 4 * g + i, n - 1)];
+    };
+    // chunk c into S, given the ring offsets fo of chunk c
+    auto load = [&](Ops &S, int c, const long long (&fo)[4]) {
+        c = min(c, JC - 1);
+        load_offsets(S.fo, c + 1);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) S.v[i] = gload(fbuf + (fo[i] + m) * (long long)K + kgl);
+        if constexpr (MODE == kFP32) {
+#pragma unroll
+            for (int ks = 0; ks < kChunk / 4; ++ks) S.af[ks] = Af[c * kChunk * 16 + ks * 64];
+        } else {
+            S.ah = Ah[c * 64];
+            S.al = Al[c * 64];
+        }
+    };
+    // stage chunk c from `cur`, load chunk c + 1 into `nxt`, then the products of chunk c
+    auto step = [&](Ops &cur, Ops &nxt, int c) {
+        const int buf = c & 1;
+        float re[4], im[4];
+        bool ok[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int j = j0 + kChunk * c + 4 * g + i;
-            v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (kok && j < n) v[i] = gload(fbuf + (as_global(P.fboff)[j] + m) * (long long)K + kgl);
+            ok[i] = kok && j0 + kChunk * c + 4 * g + i < n;
+            re[i] = ok[i] ? cur.v[i].x : 0.f;
+            im[i] = ok[i] ? cur.v[i].y : 0.f;
         }
-    };
-    float af[8], afn[8];
-    V8of<MODE> ah, al, ahn, aln;
-    auto load_lambda = [&](int c, float (&f)[8], V8of<MODE> &h, V8of<MODE> &lo) {
-        if constexpr (MODE == kFP32) {
-            const SHT_GLOBAL float *A = as_global(P.ana_f32) + as_global(P.ana_off)[m] +
-                                        ((long long)(p * nt + lt) * (JC * kChunk) + c * kChunk) * 16;
+        stage4<MODE>(sm, buf, 0, kk, g, re, im);
 #pragma unroll
-            for (int ks = 0; ks < kChunk / 4; ++ks) f[ks] = A[(ks * 4 + grp) * 16 + row];
-        } else {
-            const long long ao = as_global(P.ana_off)[m];
-            const SHT_GLOBAL V8of<MODE> *Ah =
-                as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_hi) + ao));
-            const SHT_GLOBAL V8of<MODE> *Al =
-                as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_lo) + ao));
-            const long long fi = ((long long)(p * nt + lt) * JC + c) * 64 + lane;
-            h = Ah[fi];
-            lo = Al[fi];
+        for (int i = 0; i < 4; ++i) {
+            re[i] = ok[i] ? cur.v[i].z : 0.f;
+            im[i] = ok[i] ? cur.v[i].w : 0.f;
         }
+        stage4<MODE>(sm, buf, 1, kk, g, re, im);
+        load(nxt, c + 1, cur.fo);
+        __syncthreads();
+        mma_chunk<MODE, 4>(sm, buf, p, row, grp, cur.af, cur.ah, cur.al, acc);
     };
 
-    if (JC > 0) {
-        load_fourier(0);
-        load_lambda(0, afn, ahn, aln);
-    }
-    for (int c = 0; c < JC; ++c) {
-        const int buf = c & 1;
-        {
-            float re[4], im[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                re[i] = v[i].x;
-                im[i] = v[i].y;
-            }
-            stage4<MODE>(sm, buf, 0, kk, g, re, im);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                re[i] = v[i].z;
-                im[i] = v[i].w;
-            }
-            stage4<MODE>(sm, buf, 1, kk, g, re, im);
+    if (JC > 0) {  // no rings for this order: the coefficients are zero
+        Ops S0, S1;
+        long long fo0[4];
+        load_offsets(fo0, 0);
+        load(S0, 0, fo0);
+        int c = 0;
+        for (; c + 1 < JC; c += 2) {  // straight-line body, as in the synthesis
+            step(S0, S1, c);
+            step(S1, S0, c + 1);
         }
-        if (c + 1 < JC) load_fourier(c + 1);
-#pragma unroll
-        for (int ks = 0; ks < 8; ++ks) af[ks] = afn[ks];
-        ah = ahn;
-        al = aln;
-        if (c + 1 < JC) load_lambda(c + 1, afn, ahn, aln);
-        __syncthreads();
-        mma_chunk<MODE, 4>(sm, buf, p, row, grp, af, ah, al, acc);
+        if (c < JC) step(S0, S1, c);
     }
     __syncthreads();
     // stage the 64-degree x 32-column output tile, then write l-contiguous columns
