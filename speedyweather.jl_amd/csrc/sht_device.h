@@ -95,6 +95,17 @@ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x
 __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
 // Exact x / d for 0 <= x < 2^16 and 1 <= d < 2^16 by a multiply-high with ceil(2^32 / d).
+// Lane permutations within rows of 16 lanes by DPP moves (no LDS round trip, unlike
+// __shfl_xor): the value of lane i ^ 1, of lane i ^ 2, and of lane 7 - (i & 7) within each
+// group of 8 lanes.  Every lane of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float lane_xor1(float x) { return dpp_mov<0xB1>(x); }     // quad_perm [1,0,3,2]
+__device__ __forceinline__ float lane_xor2(float x) { return dpp_mov<0x4E>(x); }     // quad_perm [2,3,0,1]
+__device__ __forceinline__ float lane_mirror8(float x) { return dpp_mov<0x141>(x); }  // row_half_mirror
+
 struct FastDiv {
     unsigned mul;
     int d;

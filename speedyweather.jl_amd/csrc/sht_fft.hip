@@ -404,7 +404,7 @@ __device__ inline void mfma_prime_rounds(float2 *buf, const float2 *tw, int N, i
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const float a1 = acc.t1[m][n][i], a2 = acc.t2[m][n][i];
-                    const float p1 = __shfl_xor(a1, 1), p2 = __shfl_xor(a2, 1);
+                    const float p1 = lane_xor1(a1), p2 = lane_xor1(a2);
                     const int s = mt * 16 + 4 * grp + i;
                     if (s > h) continue;
                     const float T1re = reim ? p1 : a1, T1im = reim ? a1 : p1;

@@ -53,9 +53,10 @@ __device__ inline void stage4(unsigned char *sm, int buf, int p, int kk, int g, 
             float s = 1.f, sinv = 1.f;
             if constexpr (MODE == kFP16x3) {
                 float amax = fmaxf(fmaxf(fabsf(x[0]), fabsf(x[1])), fmaxf(fabsf(x[2]), fabsf(x[3])));
-                amax = fmaxf(amax, __shfl_xor(amax, 1));
-                amax = fmaxf(amax, __shfl_xor(amax, 2));
-                amax = fmaxf(amax, __shfl_xor(amax, 4));
+                // maximum over the 8 lanes of the column group (lanes 8q .. 8q + 7)
+                amax = fmaxf(amax, lane_xor1(amax));
+                amax = fmaxf(amax, lane_xor2(amax));
+                amax = fmaxf(amax, lane_mirror8(amax));
                 fp16_scale(amax, s, sinv);
             }
             u16x4 hi, lo;
@@ -231,8 +232,24 @@ __global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *
 
     // straight-line loop body (a conditional second step would make the waits at the loop
     // head cover every outstanding load)
+    // Per-ring epilogue factors of this block's 64 rings (Fourier row offset, FP16 row scale,
+    // 1 / cos(lat)), loaded before the first chunk and kept in LDS until the epilogue.
+    __shared__ long long ep_row[64];
+    __shared__ float ep_scale[64];
+    long long e_row = 0;
+    float e_sc = 1.f, e_ci = 1.f;
+    if (w == 0) {
+        const int r = min(64 * T.blk + lane, J - 1);  // ring j0 + r
+        e_row = as_global(P.fboff)[j0 + r] + m;
+        if constexpr (MODE == kFP16x3) e_sc = as_global(P.syn_sinv)[as_global(P.syn_soff)[m] + r];
+        if (unscale) e_ci = as_global(P.coslat_inv)[j0 + r];
+    }
     Ops S0, S1;
     load(S0, 0);
+    if (w == 0) {
+        ep_row[lane] = e_row;
+        ep_scale[lane] = e_sc * e_ci;
+    }
     int c = 0;
     for (; c + 1 < nch; c += 2) {
         step(S0, S1, c);
@@ -251,23 +268,14 @@ __global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *
         for (int i = 0; i < 4; ++i) {
             const int r = 4 * grp + i;
             const int j = j0 + 16 * tile + r;
-            float e = acc[0][cb][i], o = acc[1][cb][i];
-            if constexpr (MODE == kFP16x3) {
-                const float sa = as_global(P.syn_sinv)[as_global(P.syn_soff)[m] + 16 * tile + r];
-                e *= sa;
-                o *= sa;
-            }
-            float nv = e + o, sv = e - o;
-            if (unscale && j < n) {
-                const float ci = as_global(P.coslat_inv)[j];
-                nv *= ci;
-                sv *= ci;
-            }
-            const float n_other = __shfl_xor(nv, 1);
-            const float s_other = __shfl_xor(sv, 1);
+            const int rb = 16 * w + r;  // ring within the block
+            const float sc = ep_scale[rb];
+            const float e = acc[0][cb][i] * sc, o = acc[1][cb][i] * sc;
+            const float nv = e + o, sv = e - o;
+            const float n_other = lane_xor1(nv);
+            const float s_other = lane_xor1(sv);
             if (j < n && k < K) {
-                SHT_GLOBAL float2 *dst =
-                    reinterpret_cast<SHT_GLOBAL float2 *>(fbuf + (as_global(P.fboff)[j] + m) * (long long)K + k);
+                SHT_GLOBAL float2 *dst = reinterpret_cast<SHT_GLOBAL float2 *>(fbuf + ep_row[rb] * (long long)K + k);
                 if (reim == 0) {
                     gstore(dst, make_float2(nv, n_other));
                 } else {
