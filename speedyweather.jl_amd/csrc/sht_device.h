@@ -94,7 +94,6 @@ __device__ inline float2 cmul(float2 a, float2 b) {
 __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
-// Exact x / d for 0 <= x < 2^16 and 1 <= d < 2^16 by a multiply-high with ceil(2^32 / d).
 // Lane permutations within rows of 16 lanes by DPP moves (no LDS round trip, unlike
 // __shfl_xor): the value of lane i ^ 1, of lane i ^ 2, and of lane 7 - (i & 7) within each
 // group of 8 lanes.  Every lane of the wave must be active.
@@ -106,10 +105,12 @@ __device__ __forceinline__ float lane_xor1(float x) { return dpp_mov<0xB1>(x); }
 __device__ __forceinline__ float lane_xor2(float x) { return dpp_mov<0x4E>(x); }     // quad_perm [2,3,0,1]
 __device__ __forceinline__ float lane_mirror8(float x) { return dpp_mov<0x141>(x); }  // row_half_mirror
 
+// Exact x / d for 0 <= x < 2^16 and 1 <= d < 2^16 by a multiply-high with ceil(2^32 / d).
 struct FastDiv {
     unsigned mul;
     int d;
     __device__ explicit FastDiv(int dd) : mul(dd > 1 ? 0xFFFFFFFFu / (unsigned)dd + 1u : 0u), d(dd) {}
+    __device__ FastDiv(int dd, unsigned m) : mul(m), d(dd) {}  // m = fastdiv_mul(dd)
     __device__ int div(int x) const { return d == 1 ? x : (int)__umulhi((unsigned)x, mul); }
 };
 

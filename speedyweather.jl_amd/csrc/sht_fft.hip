@@ -222,11 +222,11 @@ struct PassShape {
 };
 
 template <int R, bool INV>
-__device__ void fft_pass_wave(float2 *col, const float2 *tw, int N, int l, int lane) {
+__device__ void fft_pass_wave(float2 *col, const float2 *tw, int N, const FFTPass &pd, int lane) {
     constexpr int MAXB = PassShape<R>::MAXB;
-    const int M = N / (l * R);
+    const int M = pd.M, l = pd.l;
     const int nb = N / R;  // butterflies
-    const FastDiv divM(M);
+    const FastDiv divM(M, pd.mulM);
     float2 a[MAXB][R];
 #pragma unroll
     for (int i = 0; i < MAXB; ++i) {
@@ -271,29 +271,31 @@ __device__ void fft_pass_wave(float2 *col, const float2 *tw, int N, int l, int l
 // one cos/sin operand serves NT column tiles per K-step.  The cos/sin operand of lane
 // (row s, t) is the ring's twiddle w_N^{(s t mod r) M} from LDS, its index advanced by
 // 4 s mod r per K-step; all operands are loaded two K-steps ahead of their MFMAs.
-// Butterflies read and write only their own r slots, so all tiles of a round are computed
-// before any of their outputs is written.
+// The B operands are read as x_t, x_{r-t} and combined into a_t, b_t right before their
+// MFMAs.  Butterflies read and write only their own r slots, so all tiles of a round are
+// computed before any of their outputs is written.
 template <int MT, int NT>
 struct MfmaRound {
     f4 t1[MT][NT], t2[MT][NT];
 };
 
 template <bool INV, int MT, int NT>
-__device__ inline void mfma_prime_rounds(float2 *buf, const float2 *tw, int N, int r, int M, int h, int nseq,
+__device__ inline void mfma_prime_rounds(float2 *buf, const float2 *tw, int N, const FFTPass &pd, int h, int nseq,
                                          int mt0, int mstep, int nmt, int nt0, int ntstep, int ntround, int Nt,
                                          int lane) {
+    const int r = pd.R, M = pd.M;
     const int row = lane & 15, grp = lane >> 4;
     const int Kt = (h + 3) >> 2;
     const int t0 = grp + 1;
-    const FastDiv divM(M);
+    const FastDiv divM(M, pd.mulM), divR(r, pd.mulR);
     const float *bf = reinterpret_cast<const float *>(buf);
     // cos/sin operand state per owned row tile
     int e0[MT], de[MT];
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
         const int s = min((mt0 + m * mstep) * 16 + row, r - 1);  // rows above h are discarded
-        e0[m] = ((s * t0) % r) * M;
-        de[m] = ((4 * s) % r) * M;
+        e0[m] = (s * t0 - divR.div(s * t0) * r) * M;
+        de[m] = (4 * s - divR.div(4 * s) * r) * M;
     }
     const bool last_ok = t0 + 4 * (Kt - 1) <= h;  // this lane's t in the last K-step is used
     for (int rb = 0; rb * ntround < Nt; ++rb) {
@@ -321,8 +323,9 @@ __device__ inline void mfma_prime_rounds(float2 *buf, const float2 *tw, int N, i
                 acc.t2[m][n] = f4{0.f, 0.f, 0.f, 0.f};
             }
         }
-        // Operands of even K-steps in set 0, odd ones in set 1; a set is refilled with the
-        // step two ahead right after its MFMAs, so loads are always two steps in flight.
+        // Operands (twiddles; x_t in X, x_{r-t} in Y) of even K-steps in set 0, odd ones in
+        // set 1; a set is refilled with the step two ahead right after its MFMAs, so loads
+        // are always two steps in flight.
         // Fetches past the last step re-read the last step's operands (never used): the
         // loads stay unconditional and the wait for a set never waits for the other one.
         struct Ops {
@@ -361,21 +364,21 @@ __device__ inline void mfma_prime_rounds(float2 *buf, const float2 *tw, int N, i
                 const float cs = mask ? 0.f : S.A[m].x, sn = mask ? 0.f : -S.A[m].y;
 #pragma unroll
                 for (int n = 0; n < NT; ++n) {
-                    acc.t1[m][n] = mfma_f32(cs, S.X[n], acc.t1[m][n]);
-                    acc.t2[m][n] = mfma_f32(sn, S.Y[n], acc.t2[m][n]);
+                    acc.t1[m][n] = mfma_f32(cs, S.X[n] + S.Y[n], acc.t1[m][n]);  // a_t
+                    acc.t2[m][n] = mfma_f32(sn, S.X[n] - S.Y[n], acc.t2[m][n]);  // b_t
                 }
             }
         };
         fetch(S0);
         fetch(S1);
-        for (int ks = 0; ks < Kt; ks += 2) {
+        int ks = 0;
+        for (; ks + 1 < Kt; ks += 2) {  // straight-line body: the waits stay per set
             mma(S0, ks);
             fetch(S0);
-            if (ks + 1 < Kt) {
-                mma(S1, ks + 1);
-                fetch(S1);
-            }
+            mma(S1, ks + 1);
+            fetch(S1);
         }
+        if (ks < Kt) mma(S0, ks);
         // x_0 of each column's butterfly, read before any output of the round is written
         int ob[NT];
         float2 z0[NT];
@@ -425,22 +428,9 @@ __device__ inline void mfma_prime_rounds(float2 *buf, const float2 *tw, int N, i
 }
 
 template <bool INV>
-__device__ void fft_pass_mfma_first(float2 *buf, const float2 *tw, int N, int r, int ncol, int tid) {
-    const int M = N / r;  // butterflies per column
-    const int h = (r - 1) >> 1;
-    const int nseq = ncol * M;
-    {
-        const FastDiv divH(h), divM(M);
-        for (int it = tid; it < nseq * h; it += kFFTThreads) {
-            const int sq = divH.div(it), t = it - sq * h + 1;
-            const int col = divM.div(sq), b = sq - col * M;
-            float2 *x = buf + col * N + b;
-            const float2 xt = x[M * t], xr = x[M * (r - t)];
-            x[M * t] = cadd(xt, xr);
-            x[M * (r - t)] = csub(xt, xr);
-        }
-    }
-    __syncthreads();
+__device__ void fft_pass_mfma_first(float2 *buf, const float2 *tw, int N, const FFTPass &pd, int ncol, int tid) {
+    const int h = (pd.R - 1) >> 1;
+    const int nseq = ncol * pd.M;  // butterflies over all columns
     constexpr int NW = kFFTThreads / 64;
     // the wave index is uniform per wave: keep it (and all tile ownership) scalar
     const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -449,13 +439,13 @@ __device__ void fft_pass_mfma_first(float2 *buf, const float2 *tw, int N, int r,
     if (Mt > NW) {
         // waves own row tiles wave and wave + NW; rounds of 2 column tiles
         const int nmt = wave + NW < Mt ? 2 : 1;
-        mfma_prime_rounds<INV, 2, 2>(buf, tw, N, r, M, h, nseq, wave, NW, nmt, 0, 1, 2, Nt, lane);
+        mfma_prime_rounds<INV, 2, 2>(buf, tw, N, pd, h, nseq, wave, NW, nmt, 0, 1, 2, Nt, lane);
     } else {
         // G = NW / Mt waves share a row tile, each taking every G-th column tile
         const int G = NW / Mt;
         const int grp = wave / Mt;
         if (grp < G) {
-            mfma_prime_rounds<INV, 1, 4>(buf, tw, N, r, M, h, nseq, wave % Mt, 0, 1, grp, G, 4 * G, Nt, lane);
+            mfma_prime_rounds<INV, 1, 4>(buf, tw, N, pd, h, nseq, wave % Mt, 0, 1, grp, G, 4 * G, Nt, lane);
         } else {
             // idle wave: still takes part in every round's two barriers
             const int rounds = (Nt + 4 * G - 1) / (4 * G);
@@ -469,34 +459,44 @@ __device__ void fft_pass_mfma_first(float2 *buf, const float2 *tw, int N, int r,
 
 // Runs all passes in place on buf (ncol columns of N points).  A first-pass MFMA radix
 // runs across the workgroup; every other pass runs per column, column c on wave c % 4.
+// A pass descriptor read field by field (structures do not copy through global pointers).
+__device__ __forceinline__ FFTPass load_pass(const SHT_GLOBAL FFTPass *p) {
+    const SHT_GLOBAL int *q = reinterpret_cast<const SHT_GLOBAL int *>(p);
+    FFTPass d;
+    d.R = q[0];
+    d.l = q[1];
+    d.M = q[2];
+    d.mulM = (unsigned)q[3];
+    d.mulR = (unsigned)q[4];
+    return d;
+}
+
 template <bool INV>
-__device__ __attribute__((always_inline)) void fft_run(float2 *buf, const float2 *tw, const SHT_GLOBAL int *fac, int nf, int N, int ncol,
-                        int tid) {
-    int l = 1, f = 0;
-    if (nf > 0 && fac[0] >= kMfmaMinPrime) {
-        fft_pass_mfma_first<INV>(buf, tw, N, fac[0], ncol, tid);
-        l = fac[0];
+__device__ __attribute__((always_inline)) void fft_run(float2 *buf, const float2 *tw, const SHT_GLOBAL FFTPass *passes,
+                                                        int nf, int N, int ncol, int tid) {
+    int f = 0;
+    if (nf > 0 && load_pass(passes).R >= kMfmaMinPrime) {
+        const FFTPass p0 = load_pass(passes);
+        fft_pass_mfma_first<INV>(buf, tw, N, p0, ncol, tid);
         f = 1;
     }
     const int lane = tid & 63, wave = tid >> 6;
     for (int c = wave; c < ncol; c += kFFTThreads / 64) {
         float2 *col = buf + c * N;
-        int ll = l;
         for (int g = f; g < nf; ++g) {
-            const int r = fac[g];
-            switch (r) {
-                case 8: fft_pass_wave<8, INV>(col, tw, N, ll, lane); break;
-                case 2: fft_pass_wave<2, INV>(col, tw, N, ll, lane); break;
-                case 3: fft_pass_wave<3, INV>(col, tw, N, ll, lane); break;
-                case 4: fft_pass_wave<4, INV>(col, tw, N, ll, lane); break;
-                case 5: fft_pass_wave<5, INV>(col, tw, N, ll, lane); break;
-                case 7: fft_pass_wave<7, INV>(col, tw, N, ll, lane); break;
-                case 11: fft_pass_wave<11, INV>(col, tw, N, ll, lane); break;
-                case 13: fft_pass_wave<13, INV>(col, tw, N, ll, lane); break;
-                case 17: fft_pass_wave<17, INV>(col, tw, N, ll, lane); break;
-                default: fft_pass_wave<19, INV>(col, tw, N, ll, lane); break;
+            const FFTPass pd = load_pass(passes + g);
+            switch (pd.R) {
+                case 8: fft_pass_wave<8, INV>(col, tw, N, pd, lane); break;
+                case 2: fft_pass_wave<2, INV>(col, tw, N, pd, lane); break;
+                case 3: fft_pass_wave<3, INV>(col, tw, N, pd, lane); break;
+                case 4: fft_pass_wave<4, INV>(col, tw, N, pd, lane); break;
+                case 5: fft_pass_wave<5, INV>(col, tw, N, pd, lane); break;
+                case 7: fft_pass_wave<7, INV>(col, tw, N, pd, lane); break;
+                case 11: fft_pass_wave<11, INV>(col, tw, N, pd, lane); break;
+                case 13: fft_pass_wave<13, INV>(col, tw, N, pd, lane); break;
+                case 17: fft_pass_wave<17, INV>(col, tw, N, pd, lane); break;
+                default: fft_pass_wave<19, INV>(col, tw, N, pd, lane); break;
             }
-            ll *= r;
         }
     }
 }
@@ -615,7 +615,7 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_synthesis_kernel(const Dev
         }
     }
     __syncthreads();
-    fft_run<true>(b0, tw, as_global(P.fac) + G.j * kMaxFactors, as_global(P.nfac)[G.j], N, ncol, tid);
+    fft_run<true>(b0, tw, as_global(P.passes) + G.j * kMaxFactors, as_global(P.nfac)[G.j], N, ncol, tid);
     __syncthreads();
     {
         SHT_GLOBAL float *field = as_global(static_cast<float *>(C.out));
@@ -692,7 +692,7 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_analysis_kernel(const DevP
         }
     }
     __syncthreads();
-    fft_run<false>(b0, tw, as_global(P.fac) + G.j * kMaxFactors, as_global(P.nfac)[G.j], N, ncol, tid);
+    fft_run<false>(b0, tw, as_global(P.passes) + G.j * kMaxFactors, as_global(P.nfac)[G.j], N, ncol, tid);
     __syncthreads();
     {
         SHT_GLOBAL float4 *fb = as_global(static_cast<float4 *>(C.fbuf)) + as_global(P.fboff)[G.j] * (long long)K + G.k0;

@@ -66,6 +66,16 @@ struct CallArgs {
 
 enum Stage { kLegendreSynthesis = 0, kFFTSynthesis = 1, kFFTAnalysis = 2, kLegendreAnalysis = 3 };
 
+// One pass of a ring FFT (radix R, l = product of the earlier radices, M = N / (l R)) with
+// the multipliers of FastDiv for M and R precomputed, so the kernels divide by neither.
+struct FFTPass {
+    int R, l, M;
+    unsigned mulM, mulR;
+    int pad[3];
+};
+// FastDiv multiplier of d: x / d == umulhi(x, mul) for x, d < 2^16.
+inline unsigned fastdiv_mul(int d) { return d > 1 ? 0xFFFFFFFFu / (unsigned)d + 1u : 0u; }
+
 // Device view of a plan (kernels read it through a device pointer).
 struct DevPlan {
     int n;            // rings per hemisphere
@@ -106,7 +116,7 @@ struct DevPlan {
     // FFT tables
     const float *twiddle;  // interleaved complex exp(-2 pi i k / N) for each ring
     const long long *tw_off;
-    const int *fac;        // kMaxFactors radices per ring
+    const FFTPass *passes;  // kMaxFactors passes per ring, in order
     const int *nfac;
 
     // UV_from_vordiv factors (per packed lm): x = -m/(l(l+1)), u1 = eps_lm / l,

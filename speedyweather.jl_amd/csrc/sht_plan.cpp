@@ -397,6 +397,20 @@ int factorize(int N, int *fac, const FFTPolicy &pol) {
     return r == 1 ? nf : -1;
 }
 
+// Per-ring pass descriptors of a factorisation (radix, l, M and their division multipliers).
+std::vector<sht::FFTPass> pass_table(const Geometry &g, const std::vector<int> &fac, const std::vector<int> &nfac) {
+    std::vector<sht::FFTPass> t((size_t)g.n * sht::kMaxFactors, sht::FFTPass{1, 1, 1, 0u, 0u, {0, 0, 0}});
+    for (int j = 0; j < g.n; ++j) {
+        int l = 1;
+        for (int f = 0; f < nfac[j]; ++f) {
+            const int R = fac[(size_t)j * sht::kMaxFactors + f], M = g.nlon[j] / (l * R);
+            t[(size_t)j * sht::kMaxFactors + f] = sht::FFTPass{R, l, M, sht::fastdiv_mul(M), sht::fastdiv_mul(R), {0, 0, 0}};
+            l *= R;
+        }
+    }
+    return t;
+}
+
 // FFT tables of a policy: radices per ring, modelled cost and heavy-first ring order.
 int fft_tables(const Geometry &g, const FFTPolicy &pol, std::vector<int> &fac, std::vector<int> &nfac,
                std::vector<double> &cost, std::vector<int> &order) {
@@ -662,7 +676,7 @@ int build_plan(sht_plan *pl) {
     }
     if ((rc = dev_upload(A, tw, &P.twiddle))) return rc;
     if ((rc = dev_upload(A, tw_off, &P.tw_off))) return rc;
-    if ((rc = dev_upload(A, fac, &P.fac))) return rc;
+    if ((rc = dev_upload(A, pass_table(g, fac, nfac), &P.passes))) return rc;
     if ((rc = dev_upload(A, nfac, &P.nfac))) return rc;
     if ((rc = dev_upload(A, uvx, &P.uv_x))) return rc;
     if ((rc = dev_upload(A, uv1, &P.uv_1))) return rc;
@@ -923,10 +937,11 @@ int apply_fft_policy(sht_plan *plan, const FFTPolicy &pol, void *stream) {
     std::vector<double> cost;
     int rc = fft_tables(plan->g, pol, fac, nfac, cost, order);
     if (rc) return rc;
+    const std::vector<sht::FFTPass> passes = pass_table(plan->g, fac, nfac);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     if (hipStreamSynchronize(st) != hipSuccess ||
-        hipMemcpy(const_cast<int *>(plan->P.fac), fac.data(), fac.size() * sizeof(int), hipMemcpyHostToDevice) !=
-            hipSuccess ||
+        hipMemcpy(const_cast<sht::FFTPass *>(plan->P.passes), passes.data(), passes.size() * sizeof(sht::FFTPass),
+                  hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(const_cast<int *>(plan->P.nfac), nfac.data(), nfac.size() * sizeof(int), hipMemcpyHostToDevice) !=
             hipSuccess ||
         hipMemcpy(const_cast<int *>(plan->P.ring_order), order.data(), order.size() * sizeof(int),
