@@ -10,8 +10,8 @@
 //   out = sum_q (w_N^{q u M} in_q) w_r^{q s},   w_N = exp(-/+ 2 pi i / N).
 // Every pass runs in place in one LDS buffer.  A first-pass prime >= 17 (the octahedral
 // rings nlon = 16 + 4j have prime factors up to 193 at T255) runs across the workgroup as
-// two small GEMMs on MFMA; all other passes (radices 2..19, register butterflies) run on
-// one column per wave with wave-local synchronisation only.
+// two small GEMMs on MFMA; all other passes (radices 2..13, register butterflies) run
+// across the workgroup with one thread per butterfly and group of columns.
 #include "sht_device.h"
 
 namespace sht {
@@ -91,7 +91,7 @@ __device__ inline void bfly(float2 *a) {
     }
 }
 
-// DFT of an odd prime number R of values in place (R = 11, 13, 17, 19), by the symmetric
+// DFT of an odd prime number R of values in place (R = 11, 13), by the symmetric
 // pairs a_t +- a_{R-t}; cos / sin of 2 pi k / R come from the ring's table tw at k N/R.
 template <int R, bool INV>
 __device__ inline void bfly_odd(float2 *a, const float2 *tw, int stepR) {
@@ -206,56 +206,86 @@ __device__ inline void bfly_any(float2 *a, const float2 *tw, int stepR) {
         bfly_odd<R, INV>(a, tw, stepR);
 }
 
-// Ordering of LDS accesses among the lanes of one wave (the passes below run on one
-// column per wave, with no workgroup barrier).
-__device__ inline void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// A pass of radix R on one column of N <= kFFTMaxRing points, run by one wave: each lane
-// reads all its butterflies (at most MAXB) into registers, then writes them in place.
+// A pass of radix R over all columns of N <= kFFTMaxRing points, run by the workgroup.
+// Work item = (column group of CB columns, butterfly t); items go to threads in order, so
+// neighbouring lanes take neighbouring butterflies of one column group.  The index
+// arithmetic and the twiddles of a butterfly serve all CB columns of its item.  Every
+// thread reads all its items into registers, the workgroup synchronises, then the results
+// are written in place.  Columns past the call's last column hold scratch values that are
+// transformed along and never stored.
+// Radices of 11 and more (one column per item) run as two sub-passes of two columns each,
+// so that a thread holds at most one butterfly.
 template <int R>
-struct PassShape {
-    static constexpr int MAXB = (kFFTMaxRing + R * 64 - 1) / (R * 64);
+struct PassCols {
+    static constexpr int CB = R <= 7 ? 4 : R <= 8 ? 2 : 1;  // columns per work item
+    static constexpr int SUB = R <= 8 ? 1 : 2;              // column sub-passes
+    static constexpr int NCG = kFFTCols / CB / SUB;         // column groups per sub-pass
+    static constexpr int MAXI = (kFFTMaxRing / R * NCG + kFFTThreads - 1) / kFFTThreads;
 };
 
 template <int R, bool INV>
-__device__ void fft_pass_wave(float2 *col, const float2 *tw, int N, const FFTPass &pd, int lane) {
-    constexpr int MAXB = PassShape<R>::MAXB;
+__device__ void fft_subpass_wg(float2 *buf, const float2 *tw, int N, const FFTPass &pd, int tid) {
+    constexpr int CB = PassCols<R>::CB, NCG = PassCols<R>::NCG, MAXI = PassCols<R>::MAXI;
     const int M = pd.M, l = pd.l;
-    const int nb = N / R;  // butterflies
+    const int nb = N / R;  // butterflies per column
+    const int items = nb * NCG;
     const FastDiv divM(M, pd.mulM);
-    float2 a[MAXB][R];
+    // item -> (column group, butterfly), NCG <= 4
+    auto split = [&](int it, int &cg, int &t) {
+        cg = 0;
 #pragma unroll
-    for (int i = 0; i < MAXB; ++i) {
-        const int t = lane + 64 * i;
-        if (t < nb) {
+        for (int k = 1; k < NCG; ++k) cg += it >= k * nb ? 1 : 0;
+        t = it - cg * nb;
+    };
+    float2 a[MAXI][CB][R];
+#pragma unroll
+    for (int i = 0; i < MAXI; ++i) {
+        const int it = tid + kFFTThreads * i;
+        if (it < items) {
+            int cg, t;
+            split(it, cg, t);
             const int u = divM.div(t), b = t - u * M;
-            const float2 *src = col + b + M * R * u;
+            const float2 *src = buf + cg * CB * N + b + M * R * u;
 #pragma unroll
-            for (int q = 0; q < R; ++q) a[i][q] = src[M * q];
+            for (int cc = 0; cc < CB; ++cc)
+#pragma unroll
+                for (int q = 0; q < R; ++q) a[i][cc][q] = src[cc * N + M * q];
         }
     }
-    wave_sync();
+    __syncthreads();
     const int ls = M * l;  // output stride between s
     const int stepR = N / R;
 #pragma unroll
-    for (int i = 0; i < MAXB; ++i) {
-        const int t = lane + 64 * i;
-        if (t >= nb) continue;
+    for (int i = 0; i < MAXI; ++i) {
+        const int it = tid + kFFTThreads * i;
+        if (it >= items) continue;
+        int cg, t;
+        split(it, cg, t);
         const int u = divM.div(t), b = t - u * M;
         if (u > 0) {
 #pragma unroll
-            for (int q = 1; q < R; ++q) a[i][q] = cmul(a[i][q], twid<INV>(tw, q * u * M));
-        }
-        bfly_any<R, INV>(a[i], tw, stepR);
-        float2 *dst = col + b + M * u;
+            for (int q = 1; q < R; ++q) {
+                const float2 w = twid<INV>(tw, q * u * M);
 #pragma unroll
-        for (int s = 0; s < R; ++s) dst[ls * s] = a[i][s];
+                for (int cc = 0; cc < CB; ++cc) a[i][cc][q] = cmul(a[i][cc][q], w);
+            }
+        }
+        float2 *dst = buf + cg * CB * N + b + M * u;
+#pragma unroll
+        for (int cc = 0; cc < CB; ++cc) {
+            bfly_any<R, INV>(a[i][cc], tw, stepR);
+#pragma unroll
+            for (int s = 0; s < R; ++s) dst[cc * N + ls * s] = a[i][cc][s];
+        }
     }
-    wave_sync();
+    __syncthreads();
+}
+
+template <int R, bool INV>
+__device__ void fft_pass_wg(float2 *buf, const float2 *tw, int N, const FFTPass &pd, int tid) {
+#pragma unroll
+    for (int sp = 0; sp < PassCols<R>::SUB; ++sp)
+        fft_subpass_wg<R, INV>(buf + sp * (kFFTCols / PassCols<R>::SUB) * N, tw, N, pd, tid);
 }
 
 // First pass (l = 1) of an odd prime radix r >= 17 as two small GEMMs on MFMA, in place.
@@ -457,8 +487,6 @@ __device__ void fft_pass_mfma_first(float2 *buf, const float2 *tw, int N, const 
     }
 }
 
-// Runs all passes in place on buf (ncol columns of N points).  A first-pass MFMA radix
-// runs across the workgroup; every other pass runs per column, column c on wave c % 4.
 // A pass descriptor read field by field (structures do not copy through global pointers).
 __device__ __forceinline__ FFTPass load_pass(const SHT_GLOBAL FFTPass *p) {
     const SHT_GLOBAL int *q = reinterpret_cast<const SHT_GLOBAL int *>(p);
@@ -471,6 +499,8 @@ __device__ __forceinline__ FFTPass load_pass(const SHT_GLOBAL FFTPass *p) {
     return d;
 }
 
+// Runs all passes in place on buf (kFFTCols columns of N points, the first ncol of them
+// data): a first-pass MFMA radix, then the register passes, all across the workgroup.
 template <bool INV>
 __device__ __attribute__((always_inline)) void fft_run(float2 *buf, const float2 *tw, const SHT_GLOBAL FFTPass *passes,
                                                         int nf, int N, int ncol, int tid) {
@@ -480,23 +510,17 @@ __device__ __attribute__((always_inline)) void fft_run(float2 *buf, const float2
         fft_pass_mfma_first<INV>(buf, tw, N, p0, ncol, tid);
         f = 1;
     }
-    const int lane = tid & 63, wave = tid >> 6;
-    for (int c = wave; c < ncol; c += kFFTThreads / 64) {
-        float2 *col = buf + c * N;
-        for (int g = f; g < nf; ++g) {
-            const FFTPass pd = load_pass(passes + g);
-            switch (pd.R) {
-                case 8: fft_pass_wave<8, INV>(col, tw, N, pd, lane); break;
-                case 2: fft_pass_wave<2, INV>(col, tw, N, pd, lane); break;
-                case 3: fft_pass_wave<3, INV>(col, tw, N, pd, lane); break;
-                case 4: fft_pass_wave<4, INV>(col, tw, N, pd, lane); break;
-                case 5: fft_pass_wave<5, INV>(col, tw, N, pd, lane); break;
-                case 7: fft_pass_wave<7, INV>(col, tw, N, pd, lane); break;
-                case 11: fft_pass_wave<11, INV>(col, tw, N, pd, lane); break;
-                case 13: fft_pass_wave<13, INV>(col, tw, N, pd, lane); break;
-                case 17: fft_pass_wave<17, INV>(col, tw, N, pd, lane); break;
-                default: fft_pass_wave<19, INV>(col, tw, N, pd, lane); break;
-            }
+    for (int g = f; g < nf; ++g) {
+        const FFTPass pd = load_pass(passes + g);
+        switch (pd.R) {
+            case 8: fft_pass_wg<8, INV>(buf, tw, N, pd, tid); break;
+            case 2: fft_pass_wg<2, INV>(buf, tw, N, pd, tid); break;
+            case 3: fft_pass_wg<3, INV>(buf, tw, N, pd, tid); break;
+            case 4: fft_pass_wg<4, INV>(buf, tw, N, pd, tid); break;
+            case 5: fft_pass_wg<5, INV>(buf, tw, N, pd, tid); break;
+            case 7: fft_pass_wg<7, INV>(buf, tw, N, pd, tid); break;
+            case 11: fft_pass_wg<11, INV>(buf, tw, N, pd, tid); break;
+            default: fft_pass_wg<13, INV>(buf, tw, N, pd, tid); break;  // the plan admits no other radix
         }
     }
 }

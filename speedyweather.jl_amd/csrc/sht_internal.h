@@ -18,16 +18,17 @@ constexpr int kDegPerWG = 64;
 constexpr int kChunk = 32;
 constexpr int kMaxFactors = 24;
 constexpr int kFFTThreads = 256;
-// Largest ring the FFT kernel takes, and columns per FFT workgroup (one wave each).
+// Largest ring the FFT kernel takes, and columns per FFT workgroup.
 constexpr int kFFTMaxRing = 1024;
 constexpr int kFFTCols = 4;
 // A first radix of at least kMfmaMinPrime runs as the MFMA pass; the factorisation puts
 // every prime from kMfmaMinRadix up first (the largest one), and 11 or 13 first only when
 // the tuned policy asks for it.  Later passes run as register butterflies, which exist for
-// 2, 3, 4, 5, 7, 8, 11, 13, 17, 19.
+// 2, 3, 4, 5, 7, 8, 11, 13.  Rings of nlon = 4 m, m <= 256 (the supported grids) have at
+// most one prime factor >= 17, which always takes the MFMA pass.
 constexpr int kMfmaMinPrime = 11;
 constexpr int kMfmaMinRadix = 17;
-constexpr int kMaxRegisterRadix = 19;
+constexpr int kMaxRegisterRadix = 13;
 // MFMA GEMM pass of the FFT: tiles (16 output frequencies x 16 real columns) a wave keeps
 // in flight within one round; rows up to 16 * kMfmaTilesPerWave * waves, i.e. radices up
 // to kMaxMfmaRadix (255).
