@@ -21,7 +21,8 @@ namespace sht {
 
 typedef u16 u16x4 __attribute__((ext_vector_type(4)));
 
-// LDS tiles of one chunk: [buffer][parity][real column][contraction index], padded rows
+// LDS tiles of one chunk: [buffer][parity][real column][contraction index], padded rows.
+// In the split modes the padding of each hi row holds the row's inverse FP16 scale.
 constexpr int kRowF = kChunk + 4;  // FP32 row (floats): 16-byte aligned, bank-spread
 constexpr int kRowH = kChunk + 8;  // 16-bit row: 16-byte aligned, bank-spread
 constexpr int kTileF = 2 * kColsReal * kRowF;
@@ -29,7 +30,7 @@ constexpr int kTileH = 2 * kColsReal * kRowH;
 
 template <int MODE>
 struct LegSmem {
-    static constexpr int bytes = MODE == kFP32 ? 2 * kTileF * 4 : 2 * 2 * kTileH * 2 + 2 * 2 * kColsReal * 4;
+    static constexpr int bytes = MODE == kFP32 ? 2 * kTileF * 4 : 2 * 2 * kTileH * 2;
 };
 
 // Stages 4 contraction values x (re, im) of one complex column (real columns 2kk, 2kk+1)
@@ -46,7 +47,6 @@ __device__ inline void stage4(unsigned char *sm, int buf, int p, int kk, int g, 
         typedef typename Split<MODE>::T T16;
         u16 *Bh = reinterpret_cast<u16 *>(sm) + (buf * 2 + 0) * kTileH + p * kColsReal * kRowH;
         u16 *Bl = reinterpret_cast<u16 *>(sm) + (buf * 2 + 1) * kTileH + p * kColsReal * kRowH;
-        float *Bs = reinterpret_cast<float *>(reinterpret_cast<u16 *>(sm) + 4 * kTileH) + (buf * 2 + p) * kColsReal;
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             const float *x = c == 0 ? re : im;
@@ -71,7 +71,7 @@ __device__ inline void stage4(unsigned char *sm, int buf, int p, int kk, int g, 
             const int col = 2 * kk + c;
             *reinterpret_cast<u16x4 *>(Bh + col * kRowH + 4 * g) = hi;
             *reinterpret_cast<u16x4 *>(Bl + col * kRowH + 4 * g) = lo;
-            if (g == 0) Bs[col] = sinv;
+            if (g == 0) *reinterpret_cast<float *>(Bh + col * kRowH + kChunk) = sinv;
         }
     }
 }
@@ -96,8 +96,6 @@ __device__ inline void mma_chunk(const unsigned char *sm, int buf, int p, int ro
         typedef typename Split<MODE>::V V8;
         const u16 *Bh = reinterpret_cast<const u16 *>(sm) + (buf * 2 + 0) * kTileH + p * kColsReal * kRowH;
         const u16 *Bl = reinterpret_cast<const u16 *>(sm) + (buf * 2 + 1) * kTileH + p * kColsReal * kRowH;
-        const float *Bs =
-            reinterpret_cast<const float *>(reinterpret_cast<const u16 *>(sm) + 4 * kTileH) + (buf * 2 + p) * kColsReal;
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb) {
             const int col = cb * 16 + row;
@@ -107,7 +105,7 @@ __device__ inline void mma_chunk(const unsigned char *sm, int buf, int p, int ro
                 f4 t = Split<MODE>::mma(al, bh, f4{0.f, 0.f, 0.f, 0.f});
                 t = Split<MODE>::mma(ah, bl, t);
                 t = Split<MODE>::mma(ah, bh, t);
-                acc[cb] += t * Bs[col];
+                acc[cb] += t * *reinterpret_cast<const float *>(Bh + col * kRowH + kChunk);
             } else {
                 acc[cb] = Split<MODE>::mma(al, bh, acc[cb]);
                 acc[cb] = Split<MODE>::mma(ah, bl, acc[cb]);
@@ -294,8 +292,9 @@ __global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *
 // rings of one complex column (kk = tid/8, rings 4(tid%8)..+3).  Every output coefficient
 // is written exactly once (no atomics, deterministic).
 
+// Four workgroups per CU: at most 128 registers per lane and 40 KB of LDS.
 template <int MODE, bool BATCH>
-__global__ __launch_bounds__(256) void legendre_analysis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
+__global__ __launch_bounds__(256, 4) void legendre_analysis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
                                                                 const CallArgs *__restrict__ calls,
                                                                 const Tile *__restrict__ tiles) {
     // the wave index is uniform per wave: keep it (and all tile ownership) scalar
@@ -344,7 +343,7 @@ __global__ __launch_bounds__(256) void legendre_analysis_kernel(const DevPlan *_
         float4 v[4];
         float af[8];
         V8of<MODE> ah, al;
-        long long fo[4];  // ring offsets in the Fourier buffer of the following chunk
+        int fo[4];  // first Fourier rows of the following chunk's rings (< 2^31 rows)
     };
     const long long ao = as_global(P.ana_off)[m];
     const SHT_GLOBAL float *Af =
@@ -354,17 +353,17 @@ __global__ __launch_bounds__(256) void legendre_analysis_kernel(const DevPlan *_
         as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_hi) + ao)) + fi;
     const SHT_GLOBAL V8of<MODE> *Al =
         as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_lo) + ao)) + fi;
-    auto load_offsets = [&](long long (&fo)[4], int c) {
+    auto load_offsets = [&](int (&fo)[4], int c) {
         c = min(c, JC - 1);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) fo[i] = as_global(P.fboff)[min(j0 + kChunk * c + 4 * g + i, n - 1)];
+        for (int i = 0; i < 4; ++i) fo[i] = (int)as_global(P.fboff)[min(j0 + kChunk * c + 4 * g + i, n - 1)];
     };
     // chunk c into S, given the ring offsets fo of chunk c
-    auto load = [&](Ops &S, int c, const long long (&fo)[4]) {
+    auto load = [&](Ops &S, int c, const int (&fo)[4]) {
         c = min(c, JC - 1);
         load_offsets(S.fo, c + 1);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) S.v[i] = gload(fbuf + (fo[i] + m) * (long long)K + kgl);
+        for (int i = 0; i < 4; ++i) S.v[i] = gload(fbuf + (long long)(fo[i] + m) * K + kgl);
         if constexpr (MODE == kFP32) {
 #pragma unroll
             for (int ks = 0; ks < kChunk / 4; ++ks) S.af[ks] = Af[c * kChunk * 16 + ks * 64];
@@ -398,7 +397,7 @@ __global__ __launch_bounds__(256) void legendre_analysis_kernel(const DevPlan *_
 
     if (JC > 0) {  // no rings for this order: the coefficients are zero
         Ops S0, S1;
-        long long fo0[4];
+        int fo0[4];
         load_offsets(fo0, 0);
         load(S0, 0, fo0);
         int c = 0;
