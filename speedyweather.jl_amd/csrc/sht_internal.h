@@ -11,8 +11,12 @@ constexpr int kColsComplex = 32;
 constexpr int kColsReal = 64;
 // Rings per synthesis workgroup (4 waves x 16-row MFMA tiles).
 constexpr int kRingsPerWG = 64;
-// Degrees per analysis workgroup (2 parities x 2 waves x 16 rows).
-constexpr int kDegPerWG = 64;
+// Degrees per analysis workgroup (2 parities x 2 waves x 16 rows x degree tiles per wave):
+// two tiles per wave in the split modes, where one staged (and split) chunk of Fourier rows
+// then serves 128 degrees, one in FP32 mode, where staging is cheap and registers are not.
+constexpr int ana_tiles_per_wave(int mode) { return mode == 0 ? 1 : 2; }
+constexpr int ana_degs_per_wg(int mode) { return 64 * ana_tiles_per_wave(mode); }
+constexpr int kDegPerWGMax = 128;
 // Contraction chunk staged in LDS: 32 degrees-per-parity (synthesis) or 32 rings
 // (analysis), one MFMA K-step of the split modes.
 constexpr int kChunk = 32;

@@ -76,12 +76,14 @@ __device__ inline void stage4(unsigned char *sm, int buf, int p, int kk, int g, 
     }
 }
 
-// acc += A (16 rows x 32 contraction, this lane's fragment) x B tile columns cb*16..cb*16+15
-// of parity p.  FP32: af[8] holds the lane's A values for the 8 K-steps of 4.
-template <int MODE, int NCB>
-__device__ inline void mma_chunk(const unsigned char *sm, int buf, int p, int row, int grp, const float (&af)[8],
-                                 typename Split<MODE == kFP32 ? kFP16x3 : MODE>::V ah,
-                                 typename Split<MODE == kFP32 ? kFP16x3 : MODE>::V al, f4 (&acc)[NCB]) {
+// acc[t] += A_t (16 rows x 32 contraction, this lane's fragment of row tile t < TPW) x B
+// tile columns cb*16..cb*16+15 of parity p; each B fragment is read once for all TPW row
+// tiles.  FP32: af[t][8] holds the lane's A values of tile t for the 8 K-steps of 4.
+template <int MODE, int NCB, int TPW>
+__device__ inline void mma_chunk(const unsigned char *sm, int buf, int p, int row, int grp, const float (&af)[TPW][8],
+                                 const typename Split<MODE == kFP32 ? kFP16x3 : MODE>::V (&ah)[TPW],
+                                 const typename Split<MODE == kFP32 ? kFP16x3 : MODE>::V (&al)[TPW],
+                                 f4 (&acc)[TPW][NCB]) {
     if constexpr (MODE == kFP32) {
         const float *B = reinterpret_cast<const float *>(sm) + buf * kTileF + p * kColsReal * kRowF;
 #pragma unroll
@@ -89,7 +91,8 @@ __device__ inline void mma_chunk(const unsigned char *sm, int buf, int p, int ro
 #pragma unroll
             for (int cb = 0; cb < NCB; ++cb) {
                 const float b = B[(cb * 16 + row) * kRowF + 4 * ks + grp];
-                acc[cb] = mfma_f32(af[ks], b, acc[cb]);
+#pragma unroll
+                for (int t = 0; t < TPW; ++t) acc[t][cb] = mfma_f32(af[t][ks], b, acc[t][cb]);
             }
         }
     } else {
@@ -102,14 +105,21 @@ __device__ inline void mma_chunk(const unsigned char *sm, int buf, int p, int ro
             const V8 bh = __builtin_bit_cast(V8, *reinterpret_cast<const u16x8 *>(Bh + col * kRowH + 8 * grp));
             const V8 bl = __builtin_bit_cast(V8, *reinterpret_cast<const u16x8 *>(Bl + col * kRowH + 8 * grp));
             if constexpr (MODE == kFP16x3) {
-                f4 t = Split<MODE>::mma(al, bh, f4{0.f, 0.f, 0.f, 0.f});
-                t = Split<MODE>::mma(ah, bl, t);
-                t = Split<MODE>::mma(ah, bh, t);
-                acc[cb] += t * *reinterpret_cast<const float *>(Bh + col * kRowH + kChunk);
+                const float sc = *reinterpret_cast<const float *>(Bh + col * kRowH + kChunk);
+#pragma unroll
+                for (int t = 0; t < TPW; ++t) {
+                    f4 u = Split<MODE>::mma(al[t], bh, f4{0.f, 0.f, 0.f, 0.f});
+                    u = Split<MODE>::mma(ah[t], bl, u);
+                    u = Split<MODE>::mma(ah[t], bh, u);
+                    acc[t][cb] += u * sc;
+                }
             } else {
-                acc[cb] = Split<MODE>::mma(al, bh, acc[cb]);
-                acc[cb] = Split<MODE>::mma(ah, bl, acc[cb]);
-                acc[cb] = Split<MODE>::mma(ah, bh, acc[cb]);
+#pragma unroll
+                for (int t = 0; t < TPW; ++t) {
+                    acc[t][cb] = Split<MODE>::mma(al[t], bh, acc[t][cb]);
+                    acc[t][cb] = Split<MODE>::mma(ah[t], bl, acc[t][cb]);
+                    acc[t][cb] = Split<MODE>::mma(ah[t], bh, acc[t][cb]);
+                }
             }
         }
     }
@@ -117,6 +127,22 @@ __device__ inline void mma_chunk(const unsigned char *sm, int buf, int p, int ro
 
 template <int MODE>
 using V8of = typename Split<MODE == kFP32 ? kFP16x3 : MODE>::V;
+
+// The same for one row tile.
+template <int MODE, int NCB>
+__device__ inline void mma_chunk1(const unsigned char *sm, int buf, int p, int row, int grp, const float (&af)[8],
+                                  V8of<MODE> ah, V8of<MODE> al, f4 (&acc)[NCB]) {
+    float a1[1][8];
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) a1[0][ks] = af[ks];
+    const V8of<MODE> h1[1] = {ah}, l1[1] = {al};
+    f4 c1[1][NCB];
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) c1[0][cb] = acc[cb];
+    mma_chunk<MODE, NCB, 1>(sm, buf, p, row, grp, a1, h1, l1, c1);
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) acc[cb] = c1[0][cb];
+}
 
 // ============================================================ Legendre synthesis
 //
@@ -224,7 +250,7 @@ __global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *
         if (active) {
 #pragma unroll
             for (int p = 0; p < 2; ++p)
-                mma_chunk<MODE, 4>(sm, buf, p, row, grp, cur.af[p], cur.ah[p], cur.al[p], acc[p]);
+                mma_chunk1<MODE, 4>(sm, buf, p, row, grp, cur.af[p], cur.ah[p], cur.al[p], acc[p]);
         }
     };
 
@@ -286,17 +312,21 @@ __global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *
 
 // ============================================================== Legendre analysis
 //
-// Workgroup = (order m, group of 64 degrees, 32 complex columns).  Wave w owns parity
-// p = w&1 and degree tile lt = 2*blk + (w>>1) (16 rows of l' with l = m + 2l' + p) for
-// all 64 real columns.  Rings stream in chunks of 32; a staging thread loads 4 consecutive
-// rings of one complex column (kk = tid/8, rings 4(tid%8)..+3).  Every output coefficient
-// is written exactly once (no atomics, deterministic).
+// Workgroup = (order m, group of DPW = 64 TPW degrees, 32 complex columns).  Wave w owns
+// parity p = w&1 and TPW consecutive degree tiles (16 rows of l' each, l = m + 2l' + p) for
+// all 64 real columns, so the staged Fourier rows of a chunk serve DPW degrees.  Rings
+// stream in chunks of 32; a staging thread loads 4 consecutive rings of one complex column
+// (kk = tid/8, rings 4(tid%8)..+3).  Every output coefficient is written exactly once (no
+// atomics, deterministic).
 
-// Four workgroups per CU: at most 128 registers per lane and 40 KB of LDS.
+// At least three workgroups per CU (at most 168 registers per lane).
 template <int MODE, bool BATCH>
-__global__ __launch_bounds__(256, 4) void legendre_analysis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
+__global__ __launch_bounds__(256, 3) void legendre_analysis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
                                                                 const CallArgs *__restrict__ calls,
                                                                 const Tile *__restrict__ tiles) {
+    constexpr int TPW = ana_tiles_per_wave(MODE);
+    constexpr int DPW = ana_degs_per_wg(MODE);
+    constexpr int kAnaTiles = DPW / 32;  // degree tiles per parity per workgroup
     // the wave index is uniform per wave: keep it (and all tile ownership) scalar
     const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const Tile T = tiles[blockIdx.x];
@@ -312,11 +342,11 @@ __global__ __launch_bounds__(256, 4) void legendre_analysis_kernel(const DevPlan
     const int j0 = as_global(P.j0)[m];
     const int J = n - j0;
     const int L = P.lmax - m;
-    const int NG = (L + kDegPerWG - 1) / kDegPerWG;
-    const int nt = 2 * NG;
+    const int NG = (L + DPW - 1) / DPW;
+    const int nt = kAnaTiles * NG;  // degree tiles per parity in the table
     const int JC = (J + kChunk - 1) / kChunk;
     const int p = w & 1;
-    const int lt = 2 * T.blk + (w >> 1);
+    const int lt0 = kAnaTiles * T.blk + TPW * (w >> 1);  // first degree tile of this wave
     const long long off = as_global(P.lm_off)[m];
     const int kbase = cg * kColsComplex;
     const int row = lane & 15, grp = lane >> 4;
@@ -324,16 +354,26 @@ __global__ __launch_bounds__(256, 4) void legendre_analysis_kernel(const DevPlan
     const bool kok = kbase + kk < K;
     const int kgl = kok ? kbase + kk : 0;
 
-    constexpr int OUTB = kColsComplex * (kDegPerWG + 1) * 8;
+    constexpr int OUTB = kColsComplex * (DPW + 1) * 8;
     constexpr int SMB = LegSmem<MODE>::bytes > OUTB ? LegSmem<MODE>::bytes : OUTB;
     __shared__ alignas(16) unsigned char sm[SMB];
 
-    f4 acc[4];
+    f4 acc[TPW][4];
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) acc[cb] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < TPW; ++t)
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) acc[t][cb] = f4{0.f, 0.f, 0.f, 0.f};
+    // FP16 row scales of this lane's output rows, loaded now for the epilogue
+    float rs[TPW][4];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            rs[t][i] = MODE == kFP16x3 ? as_global(P.ana_sinv)[as_global(P.ana_soff)[m] + (p * nt + lt0 + t) * 16 + 4 * grp + i]
+                                       : 1.f;
 
     // Operands of one chunk: 4 rings of one column (staging role) and this wave's Λ
-    // fragment.  Two sets alternate (no register copies between chunks); the next chunk's
+    // fragments.  Two sets alternate (no register copies between chunks); the next chunk's
     // set is loaded while the current chunk's products run.  All loads are unconditional
     // (rings past n re-read ring n - 1, absent columns column 0, the step after the last
     // chunk the last chunk again; the staging zeroes what is not data), so each wait covers
@@ -341,18 +381,21 @@ __global__ __launch_bounds__(256, 4) void legendre_analysis_kernel(const DevPlan
     // after its own, loaded ahead of its rows, so no Fourier load waits for the offset table.
     struct Ops {
         float4 v[4];
-        float af[8];
-        V8of<MODE> ah, al;
+        float af[TPW][8];
+        V8of<MODE> ah[TPW], al[TPW];
         int fo[4];  // first Fourier rows of the following chunk's rings (< 2^31 rows)
     };
     const long long ao = as_global(P.ana_off)[m];
-    const SHT_GLOBAL float *Af =
-        as_global(P.ana_f32) + ao + (long long)(p * nt + lt) * (JC * kChunk) * 16 + grp * 16 + row;
-    const long long fi = (long long)(p * nt + lt) * JC * 64 + lane;
-    const SHT_GLOBAL V8of<MODE> *Ah =
-        as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_hi) + ao)) + fi;
-    const SHT_GLOBAL V8of<MODE> *Al =
-        as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_lo) + ao)) + fi;
+    const SHT_GLOBAL float *Af[TPW];
+    const SHT_GLOBAL V8of<MODE> *Ah[TPW], *Al[TPW];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+        const int lt = lt0 + t;
+        Af[t] = as_global(P.ana_f32) + ao + (long long)(p * nt + lt) * (JC * kChunk) * 16 + grp * 16 + row;
+        const long long fi = (long long)(p * nt + lt) * JC * 64 + lane;
+        Ah[t] = as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_hi) + ao)) + fi;
+        Al[t] = as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_lo) + ao)) + fi;
+    }
     auto load_offsets = [&](int (&fo)[4], int c) {
         c = min(c, JC - 1);
 #pragma unroll
@@ -364,12 +407,15 @@ __global__ __launch_bounds__(256, 4) void legendre_analysis_kernel(const DevPlan
         load_offsets(S.fo, c + 1);
 #pragma unroll
         for (int i = 0; i < 4; ++i) S.v[i] = gload(fbuf + (long long)(fo[i] + m) * K + kgl);
-        if constexpr (MODE == kFP32) {
 #pragma unroll
-            for (int ks = 0; ks < kChunk / 4; ++ks) S.af[ks] = Af[c * kChunk * 16 + ks * 64];
-        } else {
-            S.ah = Ah[c * 64];
-            S.al = Al[c * 64];
+        for (int t = 0; t < TPW; ++t) {
+            if constexpr (MODE == kFP32) {
+#pragma unroll
+                for (int ks = 0; ks < kChunk / 4; ++ks) S.af[t][ks] = Af[t][c * kChunk * 16 + ks * 64];
+            } else {
+                S.ah[t] = Ah[t][c * 64];
+                S.al[t] = Al[t][c * 64];
+            }
         }
     };
     // stage chunk c from `cur`, load chunk c + 1 into `nxt`, then the products of chunk c
@@ -392,7 +438,7 @@ __global__ __launch_bounds__(256, 4) void legendre_analysis_kernel(const DevPlan
         stage4<MODE>(sm, buf, 1, kk, g, re, im);
         load(nxt, c + 1, cur.fo);
         __syncthreads();
-        mma_chunk<MODE, 4>(sm, buf, p, row, grp, cur.af, cur.ah, cur.al, acc);
+        mma_chunk<MODE, 4, TPW>(sm, buf, p, row, grp, cur.af, cur.ah, cur.al, acc);
     };
 
     if (JC > 0) {  // no rings for this order: the coefficients are zero
@@ -408,29 +454,28 @@ __global__ __launch_bounds__(256, 4) void legendre_analysis_kernel(const DevPlan
         if (c < JC) step(S0, S1, c);
     }
     __syncthreads();
-    // stage the 64-degree x 32-column output tile, then write l-contiguous columns
-    float2 *Out = reinterpret_cast<float2 *>(sm);  // [k][l - m - 64 blk], row kDegPerWG + 1
+    // stage the DPW-degree x 32-column output tile, then write l-contiguous columns
+    float2 *Out = reinterpret_cast<float2 *>(sm);  // [k][l - m - DPW blk], row DPW + 1
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
-        const int col = cb * 16 + row;
-        const int kc = col >> 1, reim = col & 1;
+    for (int t = 0; t < TPW; ++t)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int r = 4 * grp + i;
-            const int lq = 16 * (w >> 1) + r;  // l' - 32 blk
-            float val = acc[cb][i];
-            if constexpr (MODE == kFP16x3) val *= as_global(P.ana_sinv)[as_global(P.ana_soff)[m] + (p * nt + lt) * 16 + r];
-            float *o = reinterpret_cast<float *>(&Out[kc * (kDegPerWG + 1) + 2 * lq + p]);
-            o[reim] = val;
+        for (int cb = 0; cb < 4; ++cb) {
+            const int col = cb * 16 + row;
+            const int kc = col >> 1, reim = col & 1;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int lq = 16 * (TPW * (w >> 1) + t) + 4 * grp + i;  // l' - DPW / 2 blk
+                float *o = reinterpret_cast<float *>(&Out[kc * (DPW + 1) + 2 * lq + p]);
+                o[reim] = acc[t][cb][i] * rs[t][i];
+            }
         }
-    }
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < 8; ++it) {
+    for (int it = 0; it < kColsComplex * DPW / 256; ++it) {
         const int idx = it * 256 + tid;
-        const int kc = idx >> 6, o = idx & 63;
-        const int og = kDegPerWG * T.blk + o, kq = kbase + kc;
-        if (og < L && kq < K) gstore(coeffs + (long long)kq * ld_spec + off + og, Out[kc * (kDegPerWG + 1) + o]);
+        const int kc = idx / DPW, o = idx % DPW;
+        const int og = DPW * T.blk + o, kq = kbase + kc;
+        if (og < L && kq < K) gstore(coeffs + (long long)kq * ld_spec + off + og, Out[kc * (DPW + 1) + o]);
     }
 }
 

@@ -441,6 +441,7 @@ int build_plan(sht_plan *pl) {
     Geometry &g = pl->g;
     const int n = g.n, lmax = g.lmax, mmax = g.mmax;
     const int mode = d.precision;
+    const int adeg = sht::ana_degs_per_wg(mode);  // degrees per analysis workgroup
     std::vector<void *> &A = pl->allocs;
     DevPlan &P = pl->P;
     int rc;
@@ -463,7 +464,7 @@ int build_plan(sht_plan *pl) {
     for (int m = 0; m < mmax; ++m) {
         const int J = n - g.j0[m], L = lmax - m, Le = (L + 1) / 2;
         const int Jt = (J + 15) / 16, nch = (Le + sht::kChunk - 1) / sht::kChunk;
-        const int NG = (L + sht::kDegPerWG - 1) / sht::kDegPerWG, nt = 2 * NG;
+        const int NG = (L + adeg - 1) / adeg, nt = NG * (adeg / 32);
         const int JC = (J + sht::kChunk - 1) / sht::kChunk;
         syn_off[m] = syn_total;
         ana_off[m] = ana_total;
@@ -505,7 +506,7 @@ int build_plan(sht_plan *pl) {
         for (int jj = 0; jj < J; ++jj) legendre_order(lmax, m, cosc[j0 + jj], pmm[j0 + jj], &lam[(size_t)jj * L]);
         auto LAM = [&](int l, int j) -> double { return lam[(size_t)(j - j0) * L + (l - m)]; };
         const int Jt = (J + 15) / 16, nch = (Le + sht::kChunk - 1) / sht::kChunk;
-        const int NG = (L + sht::kDegPerWG - 1) / sht::kDegPerWG, nt = 2 * NG;
+        const int NG = (L + adeg - 1) / adeg, nt = NG * (adeg / 32);
         const int JC = (J + sht::kChunk - 1) / sht::kChunk;
         // synthesis: rows = rings, K = l' per parity
         std::vector<double> srow_scale(Jt * 16, 1.0);
@@ -602,9 +603,9 @@ int build_plan(sht_plan *pl) {
             syn_t.push_back({0, m, b, 0});
             syn_c.push_back((double)std::min(sht::kRingsPerWG, J - b * sht::kRingsPerWG) * L);
         }
-        for (int b = 0; b * sht::kDegPerWG < L; ++b) {
+        for (int b = 0; b * adeg < L; ++b) {
             ana_t.push_back({0, m, b, 0});
-            ana_c.push_back((double)std::min(sht::kDegPerWG, L - b * sht::kDegPerWG) * std::max(J, 1));
+            ana_c.push_back((double)std::min(adeg, L - b * adeg) * std::max(J, 1));
         }
     }
     auto sort_tiles = [](std::vector<Tile> &t, std::vector<double> &c) {
@@ -756,10 +757,11 @@ int sht_plan_sizes(const sht_desc *desc, sht_sizes *sizes) {
     sizes->nfreq_total = g.fb_rows;
     // Λ (two directions, padded) + scratch, estimated without building
     long long lam = 0;
+    const int adeg = sht::ana_degs_per_wg(desc->precision);
     for (int m = 0; m < g.mmax; ++m) {
         const int J = g.n - g.j0[m], L = g.lmax - m, Le = (L + 1) / 2;
         lam += 2LL * ((J + 15) / 16) * ((Le + 31) / 32) * 512;
-        lam += 2LL * 2 * ((L + 63) / 64) * ((J + 31) / 32) * 512;
+        lam += 2LL * ((L + adeg - 1) / adeg) * (adeg / 32) * ((J + 31) / 32) * 512;
     }
     sizes->device_bytes = lam * 4 + g.fb_rows * (long long)desc->nlayers * 16;
     return SHT_OK;
