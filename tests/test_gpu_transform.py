@@ -146,6 +146,26 @@ def test_k_subsets_and_strided_columns(sw, gpu):
     assert rel(back.data.cpu().numpy(), b_ref) < 1e-5
 
 
+@pytest.mark.parametrize("offset, extra", [(0, 4), (4, 4), (1, 4), (2, 6)])
+def test_grid_views_aligned_and_unaligned(sw, gpu, offset, extra):
+    """Grid views at any float offset and column stride (aligned or not to 8 or 16 bytes)
+    reproduce the oracle and leave the rest of the buffer untouched."""
+    o, a, layers, fields = case(sw, "T31", seed=5)
+    S = plan(sw, "T31", "fp32")
+    f_ref = o.synthesis(a)
+    b_ref = o.analysis(f_ref)
+    npts = S.grid.npoints
+    wide = torch.full((8, npts + extra), 7.0, device="cuda")
+    fv = sw.Field(wide[:, offset:offset + npts], S.grid)
+    sw.transform_(fv, to_dev(sw, a, S.spectrum), S)
+    back = sw.transform(fv, S)
+    torch.cuda.synchronize()
+    assert rel(fv.data.cpu().numpy(), f_ref) < 1e-5
+    assert rel(back.data.cpu().numpy(), b_ref) < 1e-5
+    outside = torch.cat([wide[:, :offset], wide[:, offset + npts:]], dim=1)
+    assert torch.all(outside == 7.0)
+
+
 def test_unscale_coslat(sw, gpu):
     o, a, layers, fields = case(sw, "T127", seed=4)
     S = plan(sw, "T127", "fp32")
