@@ -102,6 +102,20 @@ int sht_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *fie
 int sht_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void *coeffs,
                  int64_t ld_spec, int32_t K, void *stream);
 
+/* Pullbacks (adjoints) of the two transforms, replace spec2grid_pullback! and
+ * grid2spec_pullback! (SpeedyTransforms/ext/SpeedyTransformsEnzymeExt.jl:263-319).
+ * sht_spec2grid_pullback: coeffs_bar (+)= S^T field_bar for the synthesis S (with the same
+ * unscale_coslat as the forward call): forward FFT of field_bar, times 2 on the interior
+ * bins (and 1/cos lat), divided by the solid angle, forward Legendre.
+ * sht_grid2spec_pullback: field_bar (+)= A^T coeffs_bar for the analysis A: inverse
+ * Legendre, times the solid angle, halved on the interior bins, inverse FFT.
+ * accumulate = 1 adds into the output as the reference does; 0 overwrites it.  Layouts,
+ * strides and streams as in sht_synthesis / sht_analysis. */
+int sht_spec2grid_pullback(sht_plan *plan, const void *field_bar, int64_t ld_grid, void *coeffs_bar,
+                           int64_t ld_spec, int32_t K, int32_t unscale_coslat, int32_t accumulate, void *stream);
+int sht_grid2spec_pullback(sht_plan *plan, const void *coeffs_bar, int64_t ld_spec, void *field_bar,
+                           int64_t ld_grid, int32_t K, int32_t accumulate, void *stream);
+
 /* U, V = (u, v) cos(lat) spectra from vorticity and divergence, replaces
  * UV_from_vordiv!(U, V, vor, div, S; radius) (SpeedyTransforms/src/spectral_gradients.jl:
  * 309-366).  All four arrays: K columns of nharm complex floats with stride ld. */

@@ -26,14 +26,16 @@ It restates, loop structure for loop structure, the reference CPU path
   unnormalised rfft / brfft (FFTW.jl 1.10.0, not vendored), restated with numpy.fft
   (rfft; irfft*N, which like brfft ignores Im of the DC and Nyquist bins);
 * vector path: ``SpeedyTransforms/src/spectral_gradients.jl:309-366`` with the factors
-  of ``SpeedyTransforms/src/gradient_arrays.jl:4-21,98-145``.
+  of ``SpeedyTransforms/src/gradient_arrays.jl:4-21,98-145``;
+* adjoints (pullbacks): ``SpeedyTransforms/ext/SpeedyTransformsEnzymeExt.jl:33-59,263-319``.
 
 Pinning (tests/test_oracle.py): Gauss nodes/weights against numpy's leggauss and exact
 polynomial quadrature; Λ against closed forms for l <= 2, against scipy's spherical
 harmonics, discrete orthonormality and the recursion residual; the FFT edge semantics
 against a brute-force DFT; the whole transform against Y00 -> constant grid, single
 harmonic round trips, the FP64 grid->spec->grid round trip at cubic dealiasing, the
-vector path against solid-body rotation. Parity unpinned: the Condon-Shortley sign of
+vector path against solid-body rotation, the pullbacks against the dot-product identity
+with synthesis and analysis (they are their exact adjoints). Parity unpinned: the Condon-Shortley sign of
 odd-m polynomials inside AssociatedLegendrePolynomials.jl (no reference test fixes it).
 """
 from __future__ import annotations
@@ -270,6 +272,38 @@ class OracleTransform:
     def analysis(self, field: np.ndarray) -> np.ndarray:
         field = np.atleast_2d(field)
         return self.forward_legendre(*self.forward_fourier(field))
+
+    # -- adjoints (SpeedyTransforms/ext/SpeedyTransformsEnzymeExt.jl:33-59,263-319)
+    def adjoint_scale(self) -> np.ndarray:
+        """(n, 1, nfreq_max): 2 for the interior rfft bins, 1 for DC and the Nyquist bin of
+        even rings, 0 past a ring's bins (SpeedyTransformsEnzymeExt.jl:35-59)."""
+        g, n = self.grid, self.grid.nlat_half
+        s = np.zeros((n, 1, int(self.nfreq.max())))
+        for j in range(n):
+            N = int(g.nlons_half[j])
+            nf = N // 2 + 1
+            s[j, 0, :nf] = [2.0 if (0 < i < nf - 1 or (N % 2 == 1 and i > 0)) else 1.0 for i in range(nf)]
+        return s
+
+    def spec2grid_pullback(self, field_bar: np.ndarray, unscale_coslat: bool = False) -> np.ndarray:
+        """Cotangent of the coefficients of synthesis (SpeedyTransformsEnzymeExt.jl:263-290):
+        forward FFT, times the adjoint scale (and 1/cos(lat)), divided by the solid angle the
+        forward Legendre applies, then the forward Legendre."""
+        n = self.grid.nlat_half
+        dgN, dgS = self.forward_fourier(np.atleast_2d(field_bar))
+        w = self.adjoint_scale() / self.solid_angles[:n, None, None]
+        if unscale_coslat:
+            w = w * self.coslat_inv[:n, None, None]
+        return self.forward_legendre(dgN * w, dgS * w)
+
+    def grid2spec_pullback(self, coeffs_bar: np.ndarray) -> np.ndarray:
+        """Cotangent of the grid of analysis (SpeedyTransformsEnzymeExt.jl:294-319): inverse
+        Legendre, times the solid angle, divided by the adjoint scale, then the inverse FFT."""
+        n = self.grid.nlat_half
+        dfN, dfS = self.inverse_legendre(np.atleast_2d(coeffs_bar))
+        s = self.adjoint_scale()
+        w = np.where(s > 0, self.solid_angles[:n, None, None] / np.where(s > 0, s, 1.0), 0.0)
+        return self.inverse_fourier(dfN * w, dfS * w)
 
 
 # --------------------------------------------------------------- vector path (C2)

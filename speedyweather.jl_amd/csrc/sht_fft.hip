@@ -622,6 +622,15 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_synthesis_kernel(const Dev
             if (col < ncol && mm <= mt) v[u] = gload(fb + (unsigned)(mm * K + col));
         }
         __builtin_amdgcn_sched_barrier(0);  // every load in flight before the first LDS store
+        if (C.adj) {  // adjoint of the analysis (grid2spec pullback): rows times ΔΩ_j / s_m
+            const float wj = as_global(P.domega)[G.j];
+#pragma unroll
+            for (int u = 0; u < kFFTRowPasses; ++u) {
+                const int mm = m0 + kFFTRowsPerPass * u;
+                const float sw = mm == 0 || 2 * mm == N ? wj : 0.5f * wj;
+                v[u] = make_float4(v[u].x * sw, v[u].y * sw, v[u].z * sw, v[u].w * sw);
+            }
+        }
         float2 *z = b0 + col * N;
 #pragma unroll
         for (int u = 0; u < kFFTRowPasses; ++u) {
@@ -657,8 +666,13 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_synthesis_kernel(const Dev
                 const int i = u * kFFTThreads + tid;
                 if (i < N) {
                     const float2 v = z[i];
-                    fn[(unsigned)i] = v.x;
-                    fs[(unsigned)i] = v.y;
+                    if (C.accumulate) {  // pullback: add into the caller's cotangent
+                        fn[(unsigned)i] += v.x;
+                        fs[(unsigned)i] += v.y;
+                    } else {
+                        fn[(unsigned)i] = v.x;
+                        fs[(unsigned)i] = v.y;
+                    }
                 }
             }
         }
@@ -722,6 +736,8 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_analysis_kernel(const DevP
         SHT_GLOBAL float4 *fb = as_global(static_cast<float4 *>(C.fbuf)) + as_global(P.fboff)[G.j] * (long long)K + G.k0;
         const int col = tid % kFFTCols, m0 = tid / kFFTCols;
         const float2 *z = b0 + col * N;
+        // adjoint of the synthesis (spec2grid pullback): rows times s_m (1/cos lat) / ΔΩ_j
+        const float wj = C.adj ? (C.unscale ? as_global(P.coslat_inv)[G.j] : 1.f) / as_global(P.domega)[G.j] : 1.f;
 #pragma unroll
         for (int u = 0; u < kFFTRowPasses; ++u) {
             const int mm = m0 + kFFTRowsPerPass * u;
@@ -729,8 +745,9 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_analysis_kernel(const DevP
             const float2 p = z[mm];
             const float2 q = z[mm == 0 ? 0 : N - mm];
             // f_N = (Z_m + conj Z_{N-m})/2, f_S = (Z_m - conj Z_{N-m})/(2i)
-            const float nre = 0.5f * (p.x + q.x), nim = 0.5f * (p.y - q.y);
-            const float sre = 0.5f * (p.y + q.y), sim = 0.5f * (q.x - p.x);
+            const float h = C.adj ? (mm == 0 || 2 * mm == N ? 0.5f * wj : wj) : 0.5f;
+            const float nre = h * (p.x + q.x), nim = h * (p.y - q.y);
+            const float sre = h * (p.y + q.y), sim = h * (q.x - p.x);
             gstore(fb + (unsigned)(mm * K + col), make_float4(nre + sre, nim + sim, nre - sre, nim - sim));
         }
     }

@@ -67,6 +67,12 @@ struct CallArgs {
     long long ld_in;      // column strides (complex elements for spectra, floats for grids)
     long long ld_out;
     void *fbuf;           // Fourier scratch of this call
+    // Adjoint (pullback) calls: 1 = the FFT analysis scales its Fourier rows by
+    // s_m (1/cos lat) / ΔΩ_j, 2 = the FFT synthesis scales its input rows by ΔΩ_j / s_m, with
+    // s_m = 2 for interior bins and 1 for DC and Nyquist; accumulate: the last stage adds
+    // into its output instead of overwriting it.
+    int adj;
+    int accumulate;
 };
 
 enum Stage { kLegendreSynthesis = 0, kFFTSynthesis = 1, kFFTAnalysis = 2, kLegendreAnalysis = 3 };
@@ -97,6 +103,7 @@ struct DevPlan {
     const long long *ring_n;  // first grid point of ring j
     const long long *ring_s;  // first grid point of the mirrored southern ring
     const float *coslat_inv;
+    const float *domega;      // solid angle ΔΩ_j of a grid point on ring j
     const int *ring_order;    // FFT workgroup -> ring, most expensive first
 
     // per order m

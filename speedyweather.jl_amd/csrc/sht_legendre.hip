@@ -475,7 +475,15 @@ __global__ __launch_bounds__(256, 3) void legendre_analysis_kernel(const DevPlan
         const int idx = it * 256 + tid;
         const int kc = idx / DPW, o = idx % DPW;
         const int og = DPW * T.blk + o, kq = kbase + kc;
-        if (og < L && kq < K) gstore(coeffs + (long long)kq * ld_spec + off + og, Out[kc * (DPW + 1) + o]);
+        if (og < L && kq < K) {
+            SHT_GLOBAL float2 *dst = coeffs + (long long)kq * ld_spec + off + og;
+            float2 v = Out[kc * (DPW + 1) + o];
+            if (C.accumulate) {  // pullback: add into the caller's cotangent
+                const float2 u = gload(dst);
+                v = make_float2(v.x + u.x, v.y + u.y);
+            }
+            gstore(dst, v);
+        }
     }
 }
 

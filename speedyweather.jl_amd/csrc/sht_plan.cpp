@@ -652,6 +652,8 @@ int build_plan(sht_plan *pl) {
     if ((rc = dev_upload(A, g.ring_n, &P.ring_n))) return rc;
     if ((rc = dev_upload(A, g.ring_s, &P.ring_s))) return rc;
     if ((rc = dev_upload(A, coslat_f, &P.coslat_inv))) return rc;
+    std::vector<float> domega_f(dOmega.begin(), dOmega.end());
+    if ((rc = dev_upload(A, domega_f, &P.domega))) return rc;
     if ((rc = dev_upload(A, ring_order, &P.ring_order))) return rc;
     if ((rc = dev_upload(A, g.j0, &P.j0))) return rc;
     if ((rc = dev_upload(A, g.lm_off, &P.lm_off))) return rc;
@@ -829,15 +831,18 @@ static int check_call(sht_plan *plan, const void *a, const void *b, int K) {
     return SHT_OK;
 }
 
-int sht_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *field, int64_t ld_grid, int32_t K,
-                  int32_t unscale_coslat, void *stream) {
+// Spectral -> grid stages (Legendre synthesis, FFT synthesis); adj = 2 and accumulate turn
+// them into the pullback of the analysis.
+static int run_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *field, int64_t ld_grid, int32_t K,
+                         int32_t unscale_coslat, int adj, int accumulate, void *stream) {
     int rc = check_call(plan, coeffs, field, K);
     if (rc) return rc;
     if (ld_spec < plan->g.nharm) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_spec < nharm");
     if (ld_grid < plan->g.npoints) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_grid < npoints");
     DeviceGuard guard(plan->d.device);
     const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
-    const CallArgs c{0, K, unscale_coslat ? 1 : 0, plan->fft_pair, coeffs, field, ld_spec, ld_grid, plan->fbuf};
+    const CallArgs c{0, K, unscale_coslat ? 1 : 0, plan->fft_pair, coeffs, field, ld_spec, ld_grid, plan->fbuf,
+                     adj, accumulate ? 1 : 0};
     hipError_t e;
     {
         ProfScope ps(plan, SHT_KERNEL_LEGENDRE_SYNTHESIS, stream);
@@ -855,15 +860,17 @@ int sht_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *fie
     return SHT_OK;
 }
 
-int sht_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void *coeffs, int64_t ld_spec, int32_t K,
-                 void *stream) {
+// Grid -> spectral stages (FFT analysis, Legendre analysis); adj = 1 and accumulate turn
+// them into the pullback of the synthesis.
+static int run_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void *coeffs, int64_t ld_spec, int32_t K,
+                        int32_t unscale_coslat, int adj, int accumulate, void *stream) {
     int rc = check_call(plan, field, coeffs, K);
     if (rc) return rc;
     if (ld_spec < plan->g.nharm) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_spec < nharm");
     if (ld_grid < plan->g.npoints) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_grid < npoints");
     DeviceGuard guard(plan->d.device);
     const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
-    const CallArgs c{0, K, 0, 0, field, coeffs, ld_grid, ld_spec, plan->fbuf};
+    const CallArgs c{0, K, unscale_coslat ? 1 : 0, 0, field, coeffs, ld_grid, ld_spec, plan->fbuf, adj, accumulate ? 1 : 0};
     hipError_t e;
     {
         ProfScope ps(plan, SHT_KERNEL_FFT_ANALYSIS, stream);
@@ -879,6 +886,26 @@ int sht_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void *coeff
     }
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("legendre analysis launch: ") + hipGetErrorString(e));
     return SHT_OK;
+}
+
+int sht_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *field, int64_t ld_grid, int32_t K,
+                  int32_t unscale_coslat, void *stream) {
+    return run_synthesis(plan, coeffs, ld_spec, field, ld_grid, K, unscale_coslat, 0, 0, stream);
+}
+
+int sht_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void *coeffs, int64_t ld_spec, int32_t K,
+                 void *stream) {
+    return run_analysis(plan, field, ld_grid, coeffs, ld_spec, K, 0, 0, 0, stream);
+}
+
+int sht_spec2grid_pullback(sht_plan *plan, const void *field_bar, int64_t ld_grid, void *coeffs_bar, int64_t ld_spec,
+                           int32_t K, int32_t unscale_coslat, int32_t accumulate, void *stream) {
+    return run_analysis(plan, field_bar, ld_grid, coeffs_bar, ld_spec, K, unscale_coslat, 1, accumulate, stream);
+}
+
+int sht_grid2spec_pullback(sht_plan *plan, const void *coeffs_bar, int64_t ld_spec, void *field_bar, int64_t ld_grid,
+                           int32_t K, int32_t accumulate, void *stream) {
+    return run_synthesis(plan, coeffs_bar, ld_spec, field_bar, ld_grid, K, 0, 2, accumulate, stream);
 }
 
 int sht_uv_from_vordiv(sht_plan *plan, const void *vor, const void *div, void *U, void *V, int64_t ld, int32_t K,

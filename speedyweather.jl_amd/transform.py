@@ -177,6 +177,41 @@ def transform(x, S: SpectralTransform, unscale_coslat: bool = False):
     raise TypeError("transform needs a Field or a LowerTriangularArray")
 
 
+def spec2grid_pullback_(coeffs_bar: LowerTriangularArray, field_bar: Field, S: SpectralTransform,
+                        unscale_coslat: bool = False, accumulate: bool = True):
+    """Cotangent of the coefficients of the spectral->grid transform (spec2grid_pullback!,
+    SpeedyTransforms/ext/SpeedyTransformsEnzymeExt.jl:263-290): coeffs_bar += S^T field_bar
+    (accumulate=False overwrites).  Returns coeffs_bar."""
+    for x in (coeffs_bar, field_bar):
+        if not S.ismatching(x):
+            raise _mismatch(S, x)
+    if coeffs_bar.nlayers != field_bar.nlayers:
+        raise DimensionMismatch(f"coeffs K={coeffs_bar.nlayers} and field K={field_bar.nlayers} differ")
+    f2, K, ld_grid = S._check_tensor(field_bar.data, S.grid.npoints, torch.float32, "field")
+    c2, _, ld_spec = S._check_tensor(coeffs_bar.data, S.spectrum.nharm, torch.complex64, "coeffs")
+    check(lib().sht_spec2grid_pullback(S._plan, f2.data_ptr(), ld_grid, c2.data_ptr(), ld_spec, K,
+                                       int(bool(unscale_coslat)), int(bool(accumulate)), S._stream()),
+          "sht_spec2grid_pullback")
+    return coeffs_bar
+
+
+def grid2spec_pullback_(field_bar: Field, coeffs_bar: LowerTriangularArray, S: SpectralTransform,
+                        accumulate: bool = True):
+    """Cotangent of the grid of the grid->spectral transform (grid2spec_pullback!,
+    SpeedyTransforms/ext/SpeedyTransformsEnzymeExt.jl:294-319): field_bar += A^T coeffs_bar
+    (accumulate=False overwrites).  Returns field_bar."""
+    for x in (coeffs_bar, field_bar):
+        if not S.ismatching(x):
+            raise _mismatch(S, x)
+    if coeffs_bar.nlayers != field_bar.nlayers:
+        raise DimensionMismatch(f"coeffs K={coeffs_bar.nlayers} and field K={field_bar.nlayers} differ")
+    c2, K, ld_spec = S._check_tensor(coeffs_bar.data, S.spectrum.nharm, torch.complex64, "coeffs")
+    f2, _, ld_grid = S._check_tensor(field_bar.data, S.grid.npoints, torch.float32, "field")
+    check(lib().sht_grid2spec_pullback(S._plan, c2.data_ptr(), ld_spec, f2.data_ptr(), ld_grid, K,
+                                       int(bool(accumulate)), S._stream()), "sht_grid2spec_pullback")
+    return field_bar
+
+
 def UV_from_vordiv_(U: LowerTriangularArray, V: LowerTriangularArray, vor: LowerTriangularArray,
                     div: LowerTriangularArray, S: SpectralTransform, radius: float = 1.0):
     """U, V = (u, v) cos(lat) from vorticity and divergence (spectral_gradients.jl:309-366)."""

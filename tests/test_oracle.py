@@ -281,3 +281,28 @@ def test_uv_from_vordiv_vs_streamfunction():
     V_ref = dpsi_dlon + np.cos(lat) * dchi_dlat
     np.testing.assert_allclose(_eval_field(sp, U[0], lat, lon), U_ref, atol=1e-8)
     np.testing.assert_allclose(_eval_field(sp, V[0], lat, lon), V_ref, atol=1e-8)
+
+
+@pytest.mark.parametrize("kind, n, lmax, mmax", [("octahedral", 8, 12, 11), ("full", 6, 9, 9), ("octahedral", 12, 17, 16)])
+@pytest.mark.parametrize("unscale", [False, True])
+def test_pullbacks_are_adjoints(kind, n, lmax, mmax, unscale):
+    """<g_bar, S a> = Re<S^T g_bar, a> and Re<a_bar, A f> = <A^T a_bar, f> for random a, f,
+    g_bar, a_bar: the pullbacks are the exact adjoints of synthesis S and analysis A."""
+    sp = Spectrum(lmax, mmax)
+    o = OracleTransform(sp, Grid(kind, n))
+    rng = np.random.default_rng(7)
+    K = 3
+    a = rng.standard_normal((K, sp.nharm)) + 1j * rng.standard_normal((K, sp.nharm))
+    gbar = rng.standard_normal((K, o.grid.npoints))
+    lhs = np.sum(gbar * o.synthesis(a, unscale_coslat=unscale))
+    rhs = np.real(np.sum(np.conj(o.spec2grid_pullback(gbar, unscale_coslat=unscale)) * a))
+    assert abs(lhs - rhs) < 1e-11 * max(1.0, abs(lhs))
+    if not unscale:
+        f = rng.standard_normal((K, o.grid.npoints))
+        abar = rng.standard_normal((K, sp.nharm)) + 1j * rng.standard_normal((K, sp.nharm))
+        lhs = np.real(np.sum(np.conj(abar) * o.analysis(f)))
+        rhs = np.sum(o.grid2spec_pullback(abar) * f)
+        assert abs(lhs - rhs) < 1e-11 * max(1.0, abs(lhs))
+    # a planted mistake (scale 1 on the interior bins) breaks the identity
+    bad = o.forward_legendre(*[x / o.solid_angles[:n, None, None] for x in o.forward_fourier(gbar)])
+    assert abs(np.sum(gbar * o.synthesis(a)) - np.real(np.sum(np.conj(bad) * a))) > 1e-6
