@@ -122,6 +122,23 @@ int sht_grid2spec_pullback(sht_plan *plan, const void *coeffs_bar, int64_t ld_sp
 int sht_uv_from_vordiv(sht_plan *plan, const void *vor, const void *div, void *U, void *V,
                        int64_t ld, int32_t K, double radius, void *stream);
 
+/* Spectral gradient operators on the plan's spectrum (unit sphere unless radius != 1), all
+ * arrays K columns of nharm complex floats with stride ld; outputs must not alias inputs
+ * (the Laplacian may run in place).  flipsign negates, add accumulates, the radius scaling
+ * applies after adding, as in the reference.
+ * sht_divergence: out = div(u, v) (curl = 0) or curl(u, v) (curl = 1) of a vector whose
+ *   components carry the 1/cos(lat) scaling; replaces divergence! / curl!
+ *   (SpeedyTransforms/src/spectral_gradients.jl:30-125).
+ * sht_laplace: out = ∇² alms or ∇⁻² alms (inverse = 1, the l = 0 mode set to 0); replaces
+ *   ∇²! / ∇⁻²! (spectral_gradients.jl:385-466).
+ * sht_gradient: dpdx = ∂p/∂λ, dpdy = cos(lat) ∂p/∂φ; replaces ∇! (spectral_gradients.jl:483-527). */
+int sht_divergence(sht_plan *plan, const void *u, const void *v, void *out, int64_t ld, int32_t K, int32_t curl,
+                   int32_t flipsign, int32_t add, double radius, void *stream);
+int sht_laplace(sht_plan *plan, const void *alms, void *out, int64_t ld, int32_t K, int32_t inverse,
+                int32_t flipsign, int32_t add, double radius, void *stream);
+int sht_gradient(sht_plan *plan, const void *p, void *dpdx, void *dpdy, int64_t ld, int32_t K, double radius,
+                 void *stream);
+
 /* Uneven batches: many independent transforms (any mix of plans, i.e. truncations and
  * grids, layer counts and fields) run as ONE launch per stage, their work tiles merged
  * and ordered heaviest first, replayed from an explicitly built hipGraph.  No reference

@@ -27,7 +27,9 @@ It restates, loop structure for loop structure, the reference CPU path
   (rfft; irfft*N, which like brfft ignores Im of the DC and Nyquist bins);
 * vector path: ``SpeedyTransforms/src/spectral_gradients.jl:309-366`` with the factors
   of ``SpeedyTransforms/src/gradient_arrays.jl:4-21,98-145``;
-* adjoints (pullbacks): ``SpeedyTransforms/ext/SpeedyTransformsEnzymeExt.jl:33-59,263-319``.
+* adjoints (pullbacks): ``SpeedyTransforms/ext/SpeedyTransformsEnzymeExt.jl:33-59,263-319``;
+* spectral gradient operators: ``SpeedyTransforms/src/spectral_gradients.jl:30-125,385-527``
+  with ``SpeedyTransforms/src/gradient_arrays.jl:4-100,150-170``.
 
 Pinning (tests/test_oracle.py): Gauss nodes/weights against numpy's leggauss and exact
 polynomial quadrature; Λ against closed forms for l <= 2, against scipy's spherical
@@ -35,7 +37,8 @@ harmonics, discrete orthonormality and the recursion residual; the FFT edge sema
 against a brute-force DFT; the whole transform against Y00 -> constant grid, single
 harmonic round trips, the FP64 grid->spec->grid round trip at cubic dealiasing, the
 vector path against solid-body rotation, the pullbacks against the dot-product identity
-with synthesis and analysis (they are their exact adjoints). Parity unpinned: the Condon-Shortley sign of
+with synthesis and analysis (they are their exact adjoints), the gradient operators against
+finite differences in latitude of the fields they act on, evaluated at arbitrary latitudes. Parity unpinned: the Condon-Shortley sign of
 odd-m polynomials inside AssociatedLegendrePolynomials.jl (no reference test fixes it).
 """
 from __future__ import annotations
@@ -348,3 +351,84 @@ def uv_from_vordiv(spectrum: Spectrum, vor: np.ndarray, div: np.ndarray, radius:
                 U[:, i] = z * div[:, i] + u2 * up - u1 * vm
                 V[:, i] = z * vor[:, i] + u1 * dm - u2 * dp
     return U * radius, V * radius
+
+
+# ------------------------------------------------------- spectral gradient operators
+
+
+def gradient_factors(spectrum: Spectrum):
+    """Per packed (l, m), FP64: grad_y_vordiv1, grad_y_vordiv2, grad_y1, grad_y2,
+    grad_x_vordiv and m (SpeedyTransforms/src/gradient_arrays.jl:4-21,27-76,80-100): with
+    0-based l, m and eps(l, m) = sqrt((l^2 - m^2) / (4 l^2 - 1)):
+    gyvd1 = (l+1) eps(l, m), gyvd2 = l eps(l+1, m), gy1 = -(l-1) eps(l, m),
+    gy2 = (l+2) eps(l+1, m), gxvd = m; every one but gy1 is 0 on the last degree l = lmax-1."""
+    lmax, mmax = spectrum.lmax, spectrum.mmax
+
+    def eps(l, m):
+        return math.sqrt((l * l - m * m) / (4.0 * l * l - 1)) if l > 0 else 0.0
+
+    f = np.zeros((6, spectrum.nharm))
+    for m in range(mmax):
+        for l in range(m, lmax):
+            i = spectrum.offsets[m] + (l - m)
+            top = l == lmax - 1
+            f[0, i] = 0.0 if top else (l + 1) * eps(l, m)
+            f[1, i] = 0.0 if top else l * eps(l + 1, m)
+            f[2, i] = -(l - 1) * eps(l, m)
+            f[3, i] = 0.0 if top else (l + 2) * eps(l + 1, m)
+            f[4, i] = 0.0 if top else m
+            f[5, i] = m
+    return f
+
+
+def _shift(a, d):
+    """a[:, lm + d] over the packed index (zero past either end)."""
+    out = np.zeros_like(a)
+    if d > 0:
+        out[:, :-d] = a[:, d:]
+    else:
+        out[:, -d:] = a[:, :d]
+    return out
+
+
+def divergence(spectrum: Spectrum, u, v, curl: bool = False, flipsign: bool = False, add=None, radius: float = 1.0):
+    """∇⋅(u, v) or ∇×(u, v) of components carrying 1/cos(lat)
+    (SpeedyTransforms/src/spectral_gradients.jl:30-125): the divergence kernel
+    i m u - gyvd1 v[lm-1] + gyvd2 v[lm+1] (curl: on (v, u) with + gyvd1, - gyvd2), the last
+    packed element 0, then flipsign, add and the 1/radius scaling in that order."""
+    f = gradient_factors(spectrum)
+    u, v = np.atleast_2d(u), np.atleast_2d(v)
+    X, Y = (v, u) if curl else (u, v)
+    sg = 1.0 if curl else -1.0
+    val = 1j * f[4] * X + sg * (f[0] * _shift(Y, -1) - f[1] * _shift(Y, 1))
+    val[:, -1] = 0.0
+    if flipsign:
+        val = -val
+    if add is not None:
+        val = val + np.atleast_2d(add)
+        val[:, -1] = 0.0
+    return val / radius
+
+
+def laplace(spectrum: Spectrum, alms, inverse: bool = False, flipsign: bool = False, add=None, radius: float = 1.0):
+    """∇² or ∇⁻² (the l = 0 mode set to 0) by the eigenvalues -l(l+1)
+    (spectral_gradients.jl:385-420, gradient_arrays.jl:150-170), then flipsign, add, and the
+    radius scaling (1/R^2, or R^2 for the inverse)."""
+    l = spectrum.l_of.astype(np.float64)
+    e = -l * (l + 1)
+    if inverse:
+        e = np.where(l == 0, 0.0, 1.0 / np.where(l == 0, 1.0, e))
+    val = np.atleast_2d(alms) * e
+    if flipsign:
+        val = -val
+    if add is not None:
+        val = val + np.atleast_2d(add)
+    return val * (radius ** 2 if inverse else 1.0 / radius ** 2)
+
+
+def gradient(spectrum: Spectrum, p, radius: float = 1.0):
+    """dpdx = i m p, dpdy = gy1 p[lm-1] + gy2 p[lm+1] (∇!, spectral_gradients.jl:483-527),
+    both divided by the radius."""
+    f = gradient_factors(spectrum)
+    p = np.atleast_2d(p)
+    return 1j * f[5] * p / radius, (f[2] * _shift(p, -1) + f[3] * _shift(p, 1)) / radius

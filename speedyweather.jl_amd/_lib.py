@@ -27,7 +27,7 @@ EXPORTS = (
     "sht_batch_uses_graph", "sht_batch_destroy", "sht_mlp_nparams", "sht_mlp_create", "sht_mlp_set_params",
     "sht_mlp_get_params", "sht_mlp_forward", "sht_mlp_gradient", "sht_mlp_train_step", "sht_mlp_destroy",
     "sht_plan_autotune", "sht_plan_tuning", "sht_plan_set_tuning", "sht_spec2grid_pullback",
-    "sht_grid2spec_pullback",
+    "sht_grid2spec_pullback", "sht_divergence", "sht_laplace", "sht_gradient",
 )
 
 
@@ -80,6 +80,9 @@ def lib() -> C.CDLL:
         L.sht_analysis.argtypes = [vp, vp, i64, vp, i64, i32, vp]
         L.sht_spec2grid_pullback.argtypes = [vp, vp, i64, vp, i64, i32, i32, i32, vp]
         L.sht_grid2spec_pullback.argtypes = [vp, vp, i64, vp, i64, i32, i32, vp]
+        L.sht_divergence.argtypes = [vp, vp, vp, vp, i64, i32, i32, i32, i32, C.c_double, vp]
+        L.sht_laplace.argtypes = [vp, vp, vp, i64, i32, i32, i32, i32, C.c_double, vp]
+        L.sht_gradient.argtypes = [vp, vp, vp, vp, i64, i32, C.c_double, vp]
         L.sht_uv_from_vordiv.argtypes = [vp, vp, vp, vp, vp, i64, i32, C.c_double, vp]
         L.sht_gauss_legendre.argtypes = [i32, vp, vp]
         L.sht_legendre_column.argtypes = [i32, i32, C.c_double, vp]

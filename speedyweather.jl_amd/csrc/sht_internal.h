@@ -1,5 +1,6 @@
 // Internal definitions shared by the host plan code and the gfx950 kernels.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include <string>
@@ -137,6 +138,10 @@ struct DevPlan {
     const float *uv_1;
     const float *uv_2;
     const int *l_of;       // 0-based degree of packed index
+    // gradient operator factors per packed lm (gradient_arrays.jl:27-100):
+    // gy = (grad_y_vordiv1, grad_y_vordiv2, grad_y1, grad_y2), gx = (grad_x_vordiv, m)
+    const float4 *gy;
+    const float2 *gx;
 };
 
 // Records the message sht_last_error() returns on this thread.
@@ -149,6 +154,11 @@ void set_last_error(const std::string &msg);
 // group from blockIdx.x.  batch = true: tile, call and column group from tiles.
 const void *legendre_kernel(int stage, int mode, bool batch);
 const void *fft_kernel(int stage, bool batch);
+// Spectral gradient operators (one kernel, op code; flags: kOp* bits).
+enum SpectralOp { kOpDivergence = 0, kOpCurl = 1, kOpLaplace = 2, kOpGradient = 3 };
+constexpr int kOpFlip = 1, kOpAdd = 2, kOpInverse = 4;
+int launch_spectral_op(const DevPlan *P, int nharm, int op, const void *a, const void *b, void *o1, void *o2,
+                       long long ld, int K, int flags, float rscale, void *stream);
 int launch_uv_from_vordiv(const DevPlan *P, int nharm, const void *vor, const void *div, void *U,
                           void *V, long long ld, int K, float radius, void *stream);
 

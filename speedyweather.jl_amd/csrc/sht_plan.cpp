@@ -633,6 +633,8 @@ int build_plan(sht_plan *pl) {
     // ---------------------------------------------------- UV_from_vordiv factors
     std::vector<float> uvx(g.nharm), uv1(g.nharm), uv2(g.nharm);
     std::vector<int> l_of(g.nharm);
+    std::vector<float4> gyv(g.nharm);
+    std::vector<float2> gxv(g.nharm);
     auto eps = [](double l, double m) { return l > 0 ? std::sqrt((l * l - m * m) / (4.0 * l * l - 1.0)) : 0.0; };
     for (int m = 0; m < mmax; ++m)
         for (int l = m; l < lmax; ++l) {
@@ -641,6 +643,11 @@ int build_plan(sht_plan *pl) {
             uvx[i] = l > 0 ? (float)(-(double)m / ((double)l * (l + 1))) : 0.f;
             uv1[i] = l > 0 ? (float)(eps(l, m) / l) : 0.f;
             uv2[i] = l < lmax - 1 ? (float)(eps(l + 1, m) / (l + 1)) : 0.f;
+            // gradient operator factors (gradient_arrays.jl:41-47,55-62,88-95), 0-based l, m
+            const bool top = l == lmax - 1;
+            gyv[i] = make_float4(top ? 0.f : (float)((l + 1) * eps(l, m)), top ? 0.f : (float)(l * eps(l + 1, m)),
+                                 (float)(-(l - 1) * eps(l, m)), top ? 0.f : (float)((l + 2) * eps(l + 1, m)));
+            gxv[i] = make_float2(top ? 0.f : (float)m, (float)m);
         }
 
     // --------------------------------------------------------------- upload
@@ -685,6 +692,8 @@ int build_plan(sht_plan *pl) {
     if ((rc = dev_upload(A, uv1, &P.uv_1))) return rc;
     if ((rc = dev_upload(A, uv2, &P.uv_2))) return rc;
     if ((rc = dev_upload(A, l_of, &P.l_of))) return rc;
+    if ((rc = dev_upload(A, gyv, &P.gy))) return rc;
+    if ((rc = dev_upload(A, gxv, &P.gx))) return rc;
     {
         const int cgs_max = (d.nlayers + sht::kColsComplex - 1) / sht::kColsComplex;
         for (int cgs = 1; cgs <= cgs_max; ++cgs)
@@ -917,6 +926,52 @@ int sht_uv_from_vordiv(sht_plan *plan, const void *vor, const void *div, void *U
     DeviceGuard guard(plan->d.device);
     rc = sht::launch_uv_from_vordiv(plan->dP, plan->g.nharm, vor, div, U, V, ld, K, (float)radius, stream);
     if (rc) return fail(SHT_ERR_HIP, std::string("UV_from_vordiv launch: ") + hipGetErrorString((hipError_t)rc));
+    return SHT_OK;
+}
+
+int sht_divergence(sht_plan *plan, const void *u, const void *v, void *out, int64_t ld, int32_t K, int32_t curl,
+                   int32_t flipsign, int32_t add, double radius, void *stream) {
+    int rc = check_call(plan, u, v, K);
+    if (rc) return rc;
+    if (!out) return fail(SHT_ERR_ARG, "null data pointer");
+    if (out == u || out == v) return fail(SHT_ERR_ARG, "output must not alias an input");
+    if (ld < plan->g.nharm) return fail(SHT_ERR_DIM, "DimensionMismatch: ld < nharm");
+    if (!(radius > 0)) return fail(SHT_ERR_ARG, "radius must be positive");
+    DeviceGuard guard(plan->d.device);
+    const int flags = (flipsign ? sht::kOpFlip : 0) | (add ? sht::kOpAdd : 0);
+    rc = sht::launch_spectral_op(plan->dP, plan->g.nharm, curl ? sht::kOpCurl : sht::kOpDivergence, u, v, out, nullptr,
+                                 ld, K, flags, (float)(1.0 / radius), stream);
+    if (rc) return fail(SHT_ERR_HIP, std::string("divergence/curl launch: ") + hipGetErrorString((hipError_t)rc));
+    return SHT_OK;
+}
+
+int sht_laplace(sht_plan *plan, const void *alms, void *out, int64_t ld, int32_t K, int32_t inverse, int32_t flipsign,
+                int32_t add, double radius, void *stream) {
+    int rc = check_call(plan, alms, out, K);
+    if (rc) return rc;
+    if (ld < plan->g.nharm) return fail(SHT_ERR_DIM, "DimensionMismatch: ld < nharm");
+    if (!(radius > 0)) return fail(SHT_ERR_ARG, "radius must be positive");
+    DeviceGuard guard(plan->d.device);
+    const int flags = (flipsign ? sht::kOpFlip : 0) | (add ? sht::kOpAdd : 0) | (inverse ? sht::kOpInverse : 0);
+    const double r2 = radius * radius;
+    rc = sht::launch_spectral_op(plan->dP, plan->g.nharm, sht::kOpLaplace, alms, nullptr, out, nullptr, ld, K, flags,
+                                 (float)(inverse ? r2 : 1.0 / r2), stream);
+    if (rc) return fail(SHT_ERR_HIP, std::string("laplace launch: ") + hipGetErrorString((hipError_t)rc));
+    return SHT_OK;
+}
+
+int sht_gradient(sht_plan *plan, const void *p, void *dpdx, void *dpdy, int64_t ld, int32_t K, double radius,
+                 void *stream) {
+    int rc = check_call(plan, p, dpdx, K);
+    if (rc) return rc;
+    if (!dpdy) return fail(SHT_ERR_ARG, "null data pointer");
+    if (dpdx == p || dpdy == p || dpdx == dpdy) return fail(SHT_ERR_ARG, "outputs must not alias the input or each other");
+    if (ld < plan->g.nharm) return fail(SHT_ERR_DIM, "DimensionMismatch: ld < nharm");
+    if (!(radius > 0)) return fail(SHT_ERR_ARG, "radius must be positive");
+    DeviceGuard guard(plan->d.device);
+    rc = sht::launch_spectral_op(plan->dP, plan->g.nharm, sht::kOpGradient, p, nullptr, dpdx, dpdy, ld, K, 0,
+                                 (float)(1.0 / radius), stream);
+    if (rc) return fail(SHT_ERR_HIP, std::string("gradient launch: ") + hipGetErrorString((hipError_t)rc));
     return SHT_OK;
 }
 

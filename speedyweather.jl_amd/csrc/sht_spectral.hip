@@ -46,6 +46,83 @@ __global__ void uv_from_vordiv_kernel(const DevPlan *__restrict__ PP, const floa
     gstore(as_global(V_) + base + lm, make_float2(vv.x * radius, vv.y * radius));
 }
 
+// ======================================================= spectral gradient operators
+// Per packed (l, m) and column, stencils in l with the reference's factor arrays
+// (gradient_arrays.jl:4-21,27-76,80-100,150-170), in float:
+//   divergence (spectral_gradients.jl:105-125): i m_x u - gyvd1 v[lm-1] + gyvd2 v[lm+1]
+//   curl (:30-41, the divergence kernel on (v, u) with the signs of b, c swapped)
+//   Laplacian (:385-420): a * (-l(l+1)) or a / (-l(l+1)) (0 at l = 0)
+//   gradient (:483-527): dpdx = i m p, dpdy = gy1 p[lm-1] + gy2 p[lm+1]
+// Factors crossing an order boundary (l = m below, l = lmax-1 above) are zero by construction.
+// The last packed element of the divergence / curl is 0; flipsign, add and the radius
+// scaling follow the reference's order (scaling after adding).
+__global__ void spectral_op_kernel(const DevPlan *__restrict__ PP, int op, const float2 *__restrict__ a_,
+                                   const float2 *__restrict__ b_, float2 *__restrict__ o1_, float2 *__restrict__ o2_,
+                                   long long ld, int flags, float rscale) {
+    const DevPlan &P = *PP;
+    const int lm = blockIdx.x * blockDim.x + threadIdx.x;
+    const int last = P.nharm - 1;
+    if (lm > last) return;
+    const long long base = (long long)blockIdx.y * ld;
+    const SHT_GLOBAL float2 *a = as_global(a_) + base;
+    SHT_GLOBAL float2 *o1 = as_global(o1_) + base;
+    const bool flip = flags & kOpFlip, add = flags & kOpAdd;
+    float2 val;
+    if (op == kOpDivergence || op == kOpCurl) {
+        if (lm == last) {
+            val = make_float2(0.f, 0.f);
+        } else {
+            const SHT_GLOBAL float2 *b = as_global(b_) + base;
+            const SHT_GLOBAL float2 *X = op == kOpCurl ? b : a;  // zonal operand
+            const SHT_GLOBAL float2 *Y = op == kOpCurl ? a : b;  // meridional operand
+            const float4 gy = gload(as_global(P.gy) + lm);       // (gyvd1, gyvd2, gy1, gy2)
+            const float mx = gload(as_global(P.gx) + lm).x;
+            const float2 z = gload(X + lm);
+            const float2 ym = lm > 0 ? gload(Y + lm - 1) : make_float2(0.f, 0.f);
+            const float2 yp = gload(Y + lm + 1);
+            const float sg = op == kOpCurl ? 1.f : -1.f;
+            val = make_float2(-mx * z.y + sg * (gy.x * ym.x - gy.y * yp.x), mx * z.x + sg * (gy.x * ym.y - gy.y * yp.y));
+            if (flip) val = make_float2(-val.x, -val.y);
+            if (add) {
+                const float2 o = gload(o1 + lm);
+                val = make_float2(o.x + val.x, o.y + val.y);
+            }
+        }
+    } else if (op == kOpLaplace) {
+        const int l = as_global(P.l_of)[lm];
+        float e = -(float)(l * (l + 1));
+        if (flags & kOpInverse) e = l == 0 ? 0.f : 1.f / e;
+        const float2 z = gload(a + lm);
+        val = make_float2(z.x * e, z.y * e);
+        if (flip) val = make_float2(-val.x, -val.y);
+        if (add) {
+            const float2 o = gload(o1 + lm);
+            val = make_float2(o.x + val.x, o.y + val.y);
+        }
+    } else {  // gradient: o1 = dpdx, o2 = dpdy
+        const float m = gload(as_global(P.gx) + lm).y;
+        const float4 gy = gload(as_global(P.gy) + lm);
+        const float2 z = gload(a + lm);
+        const float2 pm = lm > 0 ? gload(a + lm - 1) : make_float2(0.f, 0.f);
+        const float2 pp = lm < last ? gload(a + lm + 1) : make_float2(0.f, 0.f);
+        gstore(o1 + lm, make_float2(-m * z.y * rscale, m * z.x * rscale));
+        val = make_float2(gy.z * pm.x + gy.w * pp.x, gy.z * pm.y + gy.w * pp.y);
+        gstore(as_global(o2_) + base + lm, make_float2(val.x * rscale, val.y * rscale));
+        return;
+    }
+    gstore(o1 + lm, make_float2(val.x * rscale, val.y * rscale));
+}
+
+int launch_spectral_op(const DevPlan *P, int nharm, int op, const void *a, const void *b, void *o1, void *o2,
+                       long long ld, int K, int flags, float rscale, void *stream) {
+    if (nharm < 1 || K < 1 || K > 65535) return (int)hipErrorInvalidValue;
+    const dim3 grid((nharm + 255) / 256, K);
+    spectral_op_kernel<<<grid, 256, 0, as_stream(stream)>>>(P, op, static_cast<const float2 *>(a),
+                                                            static_cast<const float2 *>(b), static_cast<float2 *>(o1),
+                                                            static_cast<float2 *>(o2), ld, flags, rscale);
+    return (int)hipGetLastError();
+}
+
 int launch_uv_from_vordiv(const DevPlan *P, int nharm, const void *vor, const void *div, void *U,
                           void *V, long long ld, int K, float radius, void *stream) {
     if (nharm < 1 || K < 1 || K > 65535) return (int)hipErrorInvalidValue;

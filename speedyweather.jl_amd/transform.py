@@ -212,6 +212,63 @@ def grid2spec_pullback_(field_bar: Field, coeffs_bar: LowerTriangularArray, S: S
     return field_bar
 
 
+def _spectral_args(S: SpectralTransform, arrays, names):
+    for x in arrays:
+        if not S.ismatching(x):
+            raise _mismatch(S, x)
+    K = arrays[0].nlayers
+    if any(x.nlayers != K for x in arrays):
+        raise DimensionMismatch("spectral operator arrays must have the same number of layers")
+    ts = [S._check_tensor(x.data, S.spectrum.nharm, torch.complex64, nm) for x, nm in zip(arrays, names)]
+    if len({t[2] for t in ts}) != 1:
+        raise ValueError("spectral operator arrays must share one column stride")
+    return [t[0].data_ptr() for t in ts], K, ts[0][2]
+
+
+def divergence_(div: LowerTriangularArray, u: LowerTriangularArray, v: LowerTriangularArray, S: SpectralTransform,
+                flipsign: bool = False, add: bool = False, radius: float = 1.0):
+    """div = ∇⋅(u, v) for components carrying 1/cos(lat) (divergence!,
+    SpeedyTransforms/src/spectral_gradients.jl:44-125).  Returns div."""
+    (pu, pv, po), K, ld = _spectral_args(S, (u, v, div), ("u", "v", "div"))
+    check(lib().sht_divergence(S._plan, pu, pv, po, ld, K, 0, int(flipsign), int(add), float(radius), S._stream()),
+          "sht_divergence")
+    return div
+
+
+def curl_(curl: LowerTriangularArray, u: LowerTriangularArray, v: LowerTriangularArray, S: SpectralTransform,
+          flipsign: bool = False, add: bool = False, radius: float = 1.0):
+    """curl = ∇×(u, v) for components carrying 1/cos(lat) (curl!, spectral_gradients.jl:22-41).
+    Returns curl."""
+    (pu, pv, po), K, ld = _spectral_args(S, (u, v, curl), ("u", "v", "curl"))
+    check(lib().sht_divergence(S._plan, pu, pv, po, ld, K, 1, int(flipsign), int(add), float(radius), S._stream()),
+          "sht_divergence")
+    return curl
+
+
+def laplace_(out: LowerTriangularArray, alms: LowerTriangularArray, S: SpectralTransform, add: bool = False,
+             flipsign: bool = False, inverse: bool = False, radius: float = 1.0):
+    """out = ∇² alms, or ∇⁻² alms with inverse=True (∇²! / ∇⁻²!, spectral_gradients.jl:385-466).
+    May run in place.  Returns out."""
+    (pa, po), K, ld = _spectral_args(S, (alms, out), ("alms", "out"))
+    check(lib().sht_laplace(S._plan, pa, po, ld, K, int(inverse), int(flipsign), int(add), float(radius),
+                            S._stream()), "sht_laplace")
+    return out
+
+
+def inverse_laplace_(out: LowerTriangularArray, alms: LowerTriangularArray, S: SpectralTransform, add: bool = False,
+                     flipsign: bool = False, radius: float = 1.0):
+    """∇⁻²! (spectral_gradients.jl:456-466)."""
+    return laplace_(out, alms, S, add=add, flipsign=flipsign, inverse=True, radius=radius)
+
+
+def gradient_(dpdx: LowerTriangularArray, dpdy: LowerTriangularArray, p: LowerTriangularArray, S: SpectralTransform,
+              radius: float = 1.0):
+    """dpdx = ∂p/∂λ, dpdy = cos(lat) ∂p/∂φ (∇!, spectral_gradients.jl:483-527).  Returns (dpdx, dpdy)."""
+    (pp, px, py), K, ld = _spectral_args(S, (p, dpdx, dpdy), ("p", "dpdx", "dpdy"))
+    check(lib().sht_gradient(S._plan, pp, px, py, ld, K, float(radius), S._stream()), "sht_gradient")
+    return dpdx, dpdy
+
+
 def UV_from_vordiv_(U: LowerTriangularArray, V: LowerTriangularArray, vor: LowerTriangularArray,
                     div: LowerTriangularArray, S: SpectralTransform, radius: float = 1.0):
     """U, V = (u, v) cos(lat) from vorticity and divergence (spectral_gradients.jl:309-366)."""

@@ -7,6 +7,8 @@ import pytest
 import scipy.special
 
 from oracle.cref import CRefTransform, fft64
+from oracle.sht_oracle import divergence as o_divergence, gradient as o_gradient, gradient_factors
+from oracle.sht_oracle import laplace as o_laplace
 from oracle.sht_oracle import (Grid, OracleTransform, Spectrum, gauss_legendre, get_nlat_half,
                                get_truncation, legendre_lambda, roundup_fft, uv_from_vordiv)
 
@@ -306,3 +308,68 @@ def test_pullbacks_are_adjoints(kind, n, lmax, mmax, unscale):
     # a planted mistake (scale 1 on the interior bins) breaks the identity
     bad = o.forward_legendre(*[x / o.solid_angles[:n, None, None] for x in o.forward_fourier(gbar)])
     assert abs(np.sum(gbar * o.synthesis(a)) - np.real(np.sum(np.conj(bad) * a))) > 1e-6
+
+
+
+def _eval_order(sp, a, m, phi):
+    """sum_l a_lm lambda_lm(sin phi) for one order m at latitudes phi: (K, len(phi))."""
+    lam = legendre_lambda(sp.lmax, sp.mmax, np.sin(phi))
+    o, L = sp.offsets[m], sp.lmax - m
+    return np.atleast_2d(a)[:, o:o + L] @ lam[o:o + L]
+
+
+def _d1(F, phi, h):
+    return (-F(phi + 2 * h) + 8 * F(phi + h) - 8 * F(phi - h) + F(phi - 2 * h)) / (12 * h)
+
+
+def _d2(F, phi, h):
+    return (-F(phi + 2 * h) + 16 * F(phi + h) - 30 * F(phi) + 16 * F(phi - h) - F(phi - 2 * h)) / (12 * h * h)
+
+
+@pytest.mark.parametrize("lmax, mmax", [(12, 12), (13, 9)])
+def test_gradient_operators_against_finite_differences(lmax, mmax):
+    """The spectral operators reproduce the continuous ones on the fields they act on,
+    evaluated at latitudes off the Gaussian grid with 4th-order differences in latitude:
+    dpdx = dp/dlambda, dpdy = cos(phi) dp/dphi, Laplacian, divergence and curl of
+    (u, v) given as (u, v) / cos(phi).  Inputs have zero top degrees, so the truncation to
+    lmax degrees drops nothing."""
+    sp = Spectrum(lmax, mmax)
+    rng = np.random.default_rng(3)
+    top = sp.l_of >= lmax - 2
+
+    def rand():
+        a = rng.standard_normal((2, sp.nharm)) + 1j * rng.standard_normal((2, sp.nharm))
+        a[:, top] = 0.0
+        return a
+
+    p, u, v = rand(), rand(), rand()
+    phi, h = np.linspace(-1.2, 1.2, 7), 1e-3
+    c, sn = np.cos(phi), np.sin(phi)
+    dpdx, dpdy = o_gradient(sp, p)
+    lap = o_laplace(sp, p)
+    div, cur = o_divergence(sp, u, v), o_divergence(sp, u, v, curl=True)
+    for m in range(mmax):
+        P = lambda x: _eval_order(sp, p, m, x)
+        U = lambda x: _eval_order(sp, u, m, x)
+        V = lambda x: _eval_order(sp, v, m, x)
+        checks = [
+            (_eval_order(sp, dpdx, m, phi), 1j * m * P(phi)),
+            (_eval_order(sp, dpdy, m, phi), c * _d1(P, phi, h)),
+            (_eval_order(sp, lap, m, phi), _d2(P, phi, h) - sn / c * _d1(P, phi, h) - m * m / c ** 2 * P(phi)),
+            # div = dU/dlambda + (1/cos) d(cos^2 V)/dphi, curl = dV/dlambda - (1/cos) d(cos^2 U)/dphi
+            (_eval_order(sp, div, m, phi), 1j * m * U(phi) - 2 * sn * V(phi) + c * _d1(V, phi, h)),
+            (_eval_order(sp, cur, m, phi), 1j * m * V(phi) + 2 * sn * U(phi) - c * _d1(U, phi, h)),
+        ]
+        for got, want in checks:
+            assert np.abs(got - want).max() < 1e-6 * max(1.0, np.abs(want).max())
+    # inverse Laplacian, flipsign, add and radius as the reference orders them
+    a = rng.standard_normal((2, sp.nharm)) + 1j * rng.standard_normal((2, sp.nharm))
+    back = o_laplace(sp, o_laplace(sp, a), inverse=True)
+    assert np.allclose(back[:, sp.l_of > 0], a[:, sp.l_of > 0], rtol=1e-13, atol=1e-13)
+    assert np.all(back[:, sp.l_of == 0] == 0)
+    assert np.allclose(o_laplace(sp, a, flipsign=True, add=a, radius=2.0), (a - o_laplace(sp, a)) / 4)
+    d0 = o_divergence(sp, u, v)
+    assert np.allclose(o_divergence(sp, u, v, flipsign=True, add=a, radius=3.0)[:, :-1], (a - d0)[:, :-1] / 3)
+    # a planted mistake (the curl's signs in the divergence) fails the check above
+    bad = 1j * gradient_factors(sp)[4] * u - (-gradient_factors(sp)[0] * np.roll(v, 1, axis=1))
+    assert np.abs(_eval_order(sp, bad, 1, phi) - _eval_order(sp, div, 1, phi)).max() > 1e-3
