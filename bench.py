@@ -126,6 +126,30 @@ def bench_c2(sw, torch, steps, warmup):
                         "with unscale_coslat (K=16), FP32"}
 
 
+def bench_dense(sw, torch, steps, warmup):
+    """Round trips of the dense matrix transform (one FP32 GEMM per direction) against the
+    regular transform at small truncations, where the reference uses the dense one on GPUs:
+    T31 (nlat_half 24, K=8) and T63 (nlat_half 48, K=32), FP32."""
+    out = {}
+    for name, n, lmax, mmax, K in (("T31_K8", 24, 33, 32, 8), ("T63_K32", 48, 65, 64, 32)):
+        sp, grid = sw.Spectrum(lmax, mmax), sw.OctahedralGaussianGrid(n)
+        c = sw.LowerTriangularArray(torch.from_numpy(sw.seeded_spectrum(sp, K, 9)).to(torch.complex64).cuda(), sp)
+        f = sw.Field(torch.empty((K, grid.npoints), device="cuda"), grid)
+        b = sw.LowerTriangularArray(torch.empty_like(c.data), sp)
+        row = {}
+        for tag, cls in (("dense_ms", sw.MatrixSpectralTransform), ("sht_ms", sw.SpectralTransform)):
+            S = cls(sp, grid, nlayers=K, precision="fp32")
+
+            def step():
+                sw.transform_(f, c, S)
+                sw.transform_(b, f, S)
+            row[tag] = time_loop(torch, step, steps, warmup) * 1e3
+            S.close()
+        out[name] = row
+    out["workload"] = "round trip (spectral -> grid -> spectral), octahedral grids, FP32"
+    return out
+
+
 def bench_c4(sw, torch, steps, warmup):
     """64 independent round trips, truncations from {T31,T63,T127,T255} (octahedral), layers
     U{1..16} x fields U{1..4}, seed 1234, FP32: one batched launch per stage from a hipGraph,
@@ -277,6 +301,7 @@ def main():
     if not args.no_extra:
         extra["c2_t127_vector_path"] = bench_c2(sw, torch, args.steps, args.warmup)
         extra["c4_uneven_batch64"] = bench_c4(sw, torch, args.steps, args.warmup)
+        extra["dense_matrix_transform"] = bench_dense(sw, torch, args.steps, args.warmup)
         extra["c5_learned_correction"] = bench_c5(sw, seeds=args.c5_seeds)
 
     # accuracy against the FP64 oracle of the same algorithm (per field and degree band)

@@ -97,6 +97,14 @@ int sht_plan_sizes(const sht_desc *desc, sht_sizes *sizes);
 int sht_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *field,
                   int64_t ld_grid, int32_t K, int32_t unscale_coslat, void *stream);
 
+/* Dense matrix transform, replaces MatrixSpectralTransform(spectrum, grid; NF, nlayers)
+ * (SpeedyTransforms/src/matrix_transform.jl:56-260; the GPU default for truncations up to 64,
+ * SpeedyWeather/src/dynamics/spectral_grid.jl:343-349).  Builds a regular plan plus the two
+ * dense transform matrices (FP64-computed, stored FP32); sht_synthesis and sht_analysis then
+ * run as one FP32 GEMM each (rocBLAS), every other entry point as on a regular plan.
+ * SHT_ERR_UNSUPPORTED if the matrices would exceed 4 GiB. */
+int sht_plan_create_matrix(const sht_desc *desc, sht_plan **plan);
+
 /* grid -> spectral, replaces _transform_spec!(coeffs, field, scratch, S)
  * (spectral_transform.jl:597-631). */
 int sht_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void *coeffs,

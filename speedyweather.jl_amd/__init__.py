@@ -7,11 +7,12 @@ from . import dycore, learned
 from .mlp import ColumnMLP
 from .spectrum import LowerTriangularArray, Spectrum
 from .synthetic import seeded_grid, seeded_spectrum
-from .transform import (SpectralBatch, SpectralTransform, UV_from_vordiv_, curl_, divergence_, gradient_,
-                        grid2spec_pullback_, inverse_laplace_, laplace_, spec2grid_pullback_, transform, transform_)
+from .transform import (MatrixSpectralTransform, SpectralBatch, SpectralTransform, UV_from_vordiv_, curl_, divergence_,
+                        gradient_, grid2spec_pullback_, inverse_laplace_, laplace_, spec2grid_pullback_, transform,
+                        transform_)
 
 __all__ = [
-    "ColumnMLP", "DimensionMismatch", "SHTError", "lib", "Field", "FullGaussianGrid", "OctahedralGaussianGrid",
+    "ColumnMLP", "MatrixSpectralTransform", "DimensionMismatch", "SHTError", "lib", "Field", "FullGaussianGrid", "OctahedralGaussianGrid",
     "get_nlat_half", "get_truncation", "roundup_fft", "LowerTriangularArray", "Spectrum",
     "seeded_grid", "seeded_spectrum", "SpectralBatch", "SpectralTransform", "UV_from_vordiv_", "transform",
     "spec2grid_pullback_", "grid2spec_pullback_", "divergence_", "curl_", "laplace_", "inverse_laplace_",

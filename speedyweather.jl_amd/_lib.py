@@ -27,7 +27,7 @@ EXPORTS = (
     "sht_batch_uses_graph", "sht_batch_destroy", "sht_mlp_nparams", "sht_mlp_create", "sht_mlp_set_params",
     "sht_mlp_get_params", "sht_mlp_forward", "sht_mlp_gradient", "sht_mlp_train_step", "sht_mlp_destroy",
     "sht_plan_autotune", "sht_plan_tuning", "sht_plan_set_tuning", "sht_spec2grid_pullback",
-    "sht_grid2spec_pullback", "sht_divergence", "sht_laplace", "sht_gradient",
+    "sht_grid2spec_pullback", "sht_divergence", "sht_laplace", "sht_gradient", "sht_plan_create_matrix",
 )
 
 
@@ -74,6 +74,7 @@ def lib() -> C.CDLL:
         L = C.CDLL(LIBPATH)
         vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
         L.sht_plan_create.argtypes = [C.POINTER(ShtDesc), C.POINTER(vp)]
+        L.sht_plan_create_matrix.argtypes = [C.POINTER(ShtDesc), C.POINTER(vp)]
         L.sht_plan_destroy.argtypes = [vp]
         L.sht_plan_sizes.argtypes = [C.POINTER(ShtDesc), C.POINTER(ShtSizes)]
         L.sht_synthesis.argtypes = [vp, vp, i64, vp, i64, i32, i32, vp]

@@ -159,6 +159,8 @@ enum SpectralOp { kOpDivergence = 0, kOpCurl = 1, kOpLaplace = 2, kOpGradient = 
 constexpr int kOpFlip = 1, kOpAdd = 2, kOpInverse = 4;
 int launch_spectral_op(const DevPlan *P, int nharm, int op, const void *a, const void *b, void *o1, void *o2,
                        long long ld, int K, int flags, float rscale, void *stream);
+// field[k][p] *= scale[p] for p < npoints, k < K (column stride ld)
+int launch_scale_points(const float *scale, void *field, long long npoints, long long ld, int K, void *stream);
 int launch_uv_from_vordiv(const DevPlan *P, int nharm, const void *vor, const void *div, void *U,
                           void *V, long long ld, int K, float radius, void *stream);
 

@@ -7,6 +7,7 @@
 // Legendre polynomials (spectral_transform.jl:167-172) computed here in FP64 and stored
 // in the order the gfx950 kernels read them.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -299,6 +300,13 @@ struct sht_plan {
     void *fbuf = nullptr;
     long long device_bytes = 0;
     Profile prof;
+    // dense matrix transform (sht_plan_create_matrix): sht_synthesis / sht_analysis run as
+    // one FP32 GEMM each with these column-major matrices (complex sides interleaved re, im)
+    bool matrix = false;
+    float *mat_syn = nullptr;      // npoints x 2 nharm: columns (Re B, -Im B) of field = Re(B a)
+    float *mat_ana = nullptr;      // 2 nharm x npoints: rows (Re F, Im F) of a = F f
+    float *pt_coslat_inv = nullptr;  // 1/cos(lat) per grid point (unscale_coslat)
+    rocblas_handle blas = nullptr;
 };
 
 namespace {
@@ -342,6 +350,7 @@ struct DeviceGuard {
 
 void free_plan(sht_plan *p) {
     if (!p) return;
+    if (p->blas) (void)rocblas_destroy_handle(p->blas);
     for (hipEvent_t e : p->prof.ev) (void)hipEventDestroy(e);
     for (void *a : p->allocs) (void)hipFree(a);
     delete p;
@@ -778,6 +787,69 @@ int sht_plan_sizes(const sht_desc *desc, sht_sizes *sizes) {
     return SHT_OK;
 }
 
+// Dense transform matrices of a plan (MatrixSpectralTransform, SpeedyTransforms/src/
+// matrix_transform.jl:56-172, there probed column by column from the transform itself; here
+// the same linear maps written out in FP64): for grid point i of ring j (nlon N, northern
+// ring jn = min(j, 2n-1-j), parity sign s = (-1)^(l-m) on southern rings) and every (l, m)
+// with m <= mt_jn,
+//   B[p, lm] = s λ_lm(x_jn) w_m e^{+2π i m i/N}, w_m = 1 for m = 0 and m = N/2, 2 otherwise
+//              (brfft: Im of DC and Nyquist ignored, which B's zero imaginary part there gives)
+//   F[lm, p] = s ΔΩ_jn λ_lm(x_jn) e^{-2π i m i/N}            (rfft, then the forward Legendre)
+static int build_matrices(sht_plan *pl) {
+    const Geometry &g = pl->g;
+    const int n = g.n, lmax = g.lmax, mmax = g.mmax;
+    const long long np = g.npoints, nh = g.nharm;
+    const double bytes = 2.0 * (double)np * 2.0 * (double)nh * 4.0;
+    if (bytes > 4.0 * 1024 * 1024 * 1024)
+        return fail(SHT_ERR_UNSUPPORTED, "dense transform matrices would take " +
+                                             std::to_string((long long)(bytes / 1048576)) + " MiB (limit 4 GiB)");
+    std::vector<double> x, w;
+    gauss_legendre(2 * n, x, w);
+    std::vector<float> B((size_t)np * 2 * nh, 0.f), F((size_t)2 * nh * np, 0.f), ci(np);
+    std::vector<double> lam(nh);
+    for (int jn = 0; jn < n; ++jn) {
+        const double cosc = -x[jn], sinc = std::sqrt(std::max(0.0, 1.0 - x[jn] * x[jn]));
+        const double dO = w[jn] * 2.0 * M_PI / g.nlon[jn];
+        if (sht_legendre_column(lmax, mmax, cosc, lam.data()) != SHT_OK) return SHT_ERR_ARG;
+        const int N = g.nlon[jn];
+        for (int hemi = 0; hemi < 2; ++hemi) {
+            const long long p0 = hemi == 0 ? g.ring_n[jn] : g.ring_s[jn];
+            for (int i = 0; i < N; ++i) ci[p0 + i] = (float)(1.0 / sinc);
+            for (int m = 0; m <= g.mt[jn]; ++m) {
+                const double wm = (m == 0 || 2 * m == N) ? 1.0 : 2.0;
+                for (int i = 0; i < N; ++i) {
+                    const double ph = 2.0 * M_PI * (double)((long long)m * i % N) / N;
+                    const double c = std::cos(ph), sn = std::sin(ph);
+                    const long long p = p0 + i;
+                    for (int l = m; l < lmax; ++l) {
+                        const long long lm = g.lm_off[m] + (l - m);
+                        const double v = (hemi == 1 && ((l - m) & 1)) ? -lam[lm] : lam[lm];
+                        B[(size_t)(2 * lm) * np + p] = (float)(v * wm * c);
+                        B[(size_t)(2 * lm + 1) * np + p] = (float)(-v * wm * sn);
+                        F[(size_t)p * 2 * nh + 2 * lm] = (float)(dO * v * c);
+                        F[(size_t)p * 2 * nh + 2 * lm + 1] = (float)(-dO * v * sn);
+                    }
+                }
+            }
+        }
+    }
+    const float *d = nullptr;
+    int rc;
+    if ((rc = dev_upload(pl->allocs, B, &d))) return rc;
+    pl->mat_syn = const_cast<float *>(d);
+    if ((rc = dev_upload(pl->allocs, F, &d))) return rc;
+    pl->mat_ana = const_cast<float *>(d);
+    if ((rc = dev_upload(pl->allocs, ci, &d))) return rc;
+    pl->pt_coslat_inv = const_cast<float *>(d);
+    if (rocblas_create_handle(&pl->blas) != rocblas_status_success) {
+        pl->blas = nullptr;
+        return fail(SHT_ERR_HIP, "rocblas_create_handle failed");
+    }
+    pl->device_bytes += (long long)bytes + np * 4;
+    pl->matrix = true;
+    return SHT_OK;
+}
+
 int sht_plan_create(const sht_desc *desc, sht_plan **plan) {
     if (!plan) return fail(SHT_ERR_ARG, "null plan pointer");
     *plan = nullptr;
@@ -800,6 +872,18 @@ int sht_plan_create(const sht_desc *desc, sht_plan **plan) {
     if (rc) { free_plan(pl); return rc; }
     *plan = pl;
     return SHT_OK;
+}
+
+int sht_plan_create_matrix(const sht_desc *desc, sht_plan **plan) {
+    int rc = sht_plan_create(desc, plan);
+    if (rc) return rc;
+    DeviceGuard guard(desc->device);
+    rc = build_matrices(*plan);
+    if (rc) {
+        free_plan(*plan);
+        *plan = nullptr;
+    }
+    return rc;
 }
 
 int sht_plan_destroy(sht_plan *plan) {
@@ -850,6 +934,20 @@ static int run_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, vo
     if (ld_grid < plan->g.npoints) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_grid < npoints");
     DeviceGuard guard(plan->d.device);
     const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
+    if (plan->matrix && adj == 0 && !accumulate) {  // field = B_int a_int, one GEMM
+        const float one = 1.f, zero = 0.f;
+        if (rocblas_set_stream(plan->blas, reinterpret_cast<hipStream_t>(stream)) != rocblas_status_success ||
+            rocblas_sgemm(plan->blas, rocblas_operation_none, rocblas_operation_none, (int)plan->g.npoints, K,
+                          (int)(2 * plan->g.nharm), &one, plan->mat_syn, (int)plan->g.npoints,
+                          static_cast<const float *>(coeffs), (int)(2 * ld_spec), &zero, static_cast<float *>(field),
+                          (int)ld_grid) != rocblas_status_success)
+            return fail(SHT_ERR_HIP, "rocblas_sgemm (dense synthesis) failed");
+        if (unscale_coslat) {
+            rc = sht::launch_scale_points(plan->pt_coslat_inv, field, plan->g.npoints, ld_grid, K, stream);
+            if (rc) return fail(SHT_ERR_HIP, std::string("unscale_coslat launch: ") + hipGetErrorString((hipError_t)rc));
+        }
+        return SHT_OK;
+    }
     const CallArgs c{0, K, unscale_coslat ? 1 : 0, plan->fft_pair, coeffs, field, ld_spec, ld_grid, plan->fbuf,
                      adj, accumulate ? 1 : 0};
     hipError_t e;
@@ -879,6 +977,16 @@ static int run_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void
     if (ld_grid < plan->g.npoints) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_grid < npoints");
     DeviceGuard guard(plan->d.device);
     const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
+    if (plan->matrix && adj == 0 && !accumulate) {  // a_int = F_int f, one GEMM
+        const float one = 1.f, zero = 0.f;
+        if (rocblas_set_stream(plan->blas, reinterpret_cast<hipStream_t>(stream)) != rocblas_status_success ||
+            rocblas_sgemm(plan->blas, rocblas_operation_none, rocblas_operation_none, (int)(2 * plan->g.nharm), K,
+                          (int)plan->g.npoints, &one, plan->mat_ana, (int)(2 * plan->g.nharm),
+                          static_cast<const float *>(field), (int)ld_grid, &zero, static_cast<float *>(coeffs),
+                          (int)(2 * ld_spec)) != rocblas_status_success)
+            return fail(SHT_ERR_HIP, "rocblas_sgemm (dense analysis) failed");
+        return SHT_OK;
+    }
     const CallArgs c{0, K, unscale_coslat ? 1 : 0, 0, field, coeffs, ld_grid, ld_spec, plan->fbuf, adj, accumulate ? 1 : 0};
     hipError_t e;
     {

@@ -123,6 +123,22 @@ int launch_spectral_op(const DevPlan *P, int nharm, int op, const void *a, const
     return (int)hipGetLastError();
 }
 
+// =========================================================== per-point scaling
+__global__ void scale_points_kernel(const float *__restrict__ scale, float *__restrict__ field, long long npoints,
+                                    long long ld) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npoints) return;
+    SHT_GLOBAL float *f = as_global(field) + (long long)blockIdx.y * ld + p;
+    *f = *f * as_global(scale)[p];
+}
+
+int launch_scale_points(const float *scale, void *field, long long npoints, long long ld, int K, void *stream) {
+    if (npoints < 1 || K < 1 || K > 65535) return (int)hipErrorInvalidValue;
+    const dim3 grid((unsigned)((npoints + 255) / 256), K);
+    scale_points_kernel<<<grid, 256, 0, as_stream(stream)>>>(scale, static_cast<float *>(field), npoints, ld);
+    return (int)hipGetLastError();
+}
+
 int launch_uv_from_vordiv(const DevPlan *P, int nharm, const void *vor, const void *div, void *U,
                           void *V, long long ld, int K, float radius, void *stream) {
     if (nharm < 1 || K < 1 || K > 65535) return (int)hipErrorInvalidValue;

@@ -42,6 +42,9 @@ class SpectralTransform:
                                   self.nlayers, _lib.SHORTCUTS[LegendreShortcut],
                                   _lib.PRECISIONS[precision], self.device)
         self._plan = C.c_void_p()
+        self._create()
+
+    def _create(self):
         check(lib().sht_plan_create(C.byref(self._desc), C.byref(self._plan)), "sht_plan_create")
 
     @classmethod
@@ -120,6 +123,15 @@ class SpectralTransform:
         if isinstance(x, Field):
             return x.grid == self.grid and x.nlayers <= self.nlayers
         return False
+
+
+class MatrixSpectralTransform(SpectralTransform):
+    """Dense matrix transform (MatrixSpectralTransform, SpeedyTransforms/src/
+    matrix_transform.jl:56-260): spectral <-> grid as one FP32 GEMM each with precomputed
+    matrices; the same API as SpectralTransform (the reference's GPU default up to T64)."""
+
+    def _create(self):
+        check(lib().sht_plan_create_matrix(C.byref(self._desc), C.byref(self._plan)), "sht_plan_create_matrix")
 
 
 def _mismatch(S: SpectralTransform, x) -> DimensionMismatch:
