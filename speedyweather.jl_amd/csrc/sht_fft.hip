@@ -466,7 +466,21 @@ __device__ void fft_pass_mfma_first(float2 *buf, const float2 *tw, int N, const 
     const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int Mt = (h + 16) >> 4;         // row tiles (rows 0..h), at most 2 NW
     const int Nt = (2 * nseq + 15) >> 4;  // column tiles over (butterfly, re/im)
-    if (Mt > NW) {
+    if (Nt <= 2 && Mt <= NW) {
+        // one or two column tiles (rings of nlon = 4 r) and at most NW row tiles: each wave
+        // takes one column tile and every (NW / Nt)-th row tile, so no MFMA runs on padding
+        // columns; one round
+        const int share = NW / Nt;  // waves per column tile
+        const int col = Nt == 2 ? (wave & 1) : 0;
+        const int r0 = Nt == 2 ? (wave >> 1) : wave;
+        const int nmt = r0 < Mt ? (Mt - r0 + share - 1) / share : 0;
+        if (nmt > 0) {
+            mfma_prime_rounds<INV, 4, 1>(buf, tw, N, pd, h, nseq, r0, share, nmt, col, 1, 2, Nt, lane);
+        } else {  // idle wave: the round's two barriers
+            __syncthreads();
+            __syncthreads();
+        }
+    } else if (Mt > NW) {
         // waves own row tiles wave and wave + NW; rounds of 2 column tiles
         const int nmt = wave + NW < Mt ? 2 : 1;
         mfma_prime_rounds<INV, 2, 2>(buf, tw, N, pd, h, nseq, wave, NW, nmt, 0, 1, 2, Nt, lane);
