@@ -22,7 +22,8 @@ namespace sht {
 typedef u16 u16x4 __attribute__((ext_vector_type(4)));
 
 // LDS tiles of one chunk: [buffer][parity][real column][contraction index], padded rows.
-// In the split modes the padding of each hi row holds the row's inverse FP16 scale.
+// In the split modes the padding of each hi row holds the row's inverse FP16 scale; the
+// synthesis keeps per-ring epilogue factors in the padding of 64 further rows (ep_slot).
 constexpr int kRowF = kChunk + 4;  // FP32 row (floats): 16-byte aligned, bank-spread
 constexpr int kRowH = kChunk + 8;  // 16-bit row: 16-byte aligned, bank-spread
 constexpr int kTileF = 2 * kColsReal * kRowF;
@@ -151,8 +152,22 @@ __device__ inline void mma_chunk1(const unsigned char *sm, int buf, int p, int r
 // Degrees l = m + 2l' + p stream in chunks of 32 l' per parity; a staging thread loads 8
 // consecutive degrees of one complex column (kk = tid/8, degrees 8(tid%8)..+7).
 
+
+// Per-ring epilogue factors of ring rb < 64 of a synthesis block: 16 bytes in the padding of
+// staging row rb (parity 0, buffer 0; the FP32 row, or the lo row of the split modes), which
+// the staging and the products never touch: the Fourier row offset, then the row scale.
+template <int MODE>
+__device__ inline unsigned char *ep_slot(unsigned char *sm, int rb) {
+    if constexpr (MODE == kFP32)
+        return sm + (rb * kRowF + kChunk) * 4;
+    else
+        return sm + (kTileH + rb * kRowH + kChunk) * 2;
+}
+
+// Four workgroups per CU: at most 128 registers per lane, and the per-ring epilogue factors
+// sit in the staging rows' padding, so LDS stays at 40 KiB (36 KiB in FP32).
 template <int MODE, bool BATCH>
-__global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
+__global__ __launch_bounds__(256, 4) void legendre_synthesis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
                                                                  const CallArgs *__restrict__ calls,
                                                                  const Tile *__restrict__ tiles) {
     // the wave index is uniform per wave: keep it (and all tile ownership) scalar
@@ -258,8 +273,7 @@ __global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *
     // head cover every outstanding load)
     // Per-ring epilogue factors of this block's 64 rings (Fourier row offset, FP16 row scale,
     // 1 / cos(lat)), loaded before the first chunk and kept in LDS until the epilogue.
-    __shared__ long long ep_row[64];
-    __shared__ float ep_scale[64];
+    static_assert(kRowF - kChunk >= 4 && kRowH - kChunk >= 8, "16 bytes of row padding");
     long long e_row = 0;
     float e_sc = 1.f, e_ci = 1.f;
     if (w == 0) {
@@ -271,8 +285,9 @@ __global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *
     Ops S0, S1;
     load(S0, 0);
     if (w == 0) {
-        ep_row[lane] = e_row;
-        ep_scale[lane] = e_sc * e_ci;
+        unsigned char *slot = ep_slot<MODE>(sm, lane);
+        *reinterpret_cast<long long *>(slot) = e_row;
+        *reinterpret_cast<float *>(slot + 8) = e_sc * e_ci;
     }
     int c = 0;
     for (; c + 1 < nch; c += 2) {
@@ -293,13 +308,15 @@ __global__ __launch_bounds__(256) void legendre_synthesis_kernel(const DevPlan *
             const int r = 4 * grp + i;
             const int j = j0 + 16 * tile + r;
             const int rb = 16 * w + r;  // ring within the block
-            const float sc = ep_scale[rb];
+            const unsigned char *slot = ep_slot<MODE>(sm, rb);
+            const float sc = *reinterpret_cast<const float *>(slot + 8);
             const float e = acc[0][cb][i] * sc, o = acc[1][cb][i] * sc;
             const float nv = e + o, sv = e - o;
             const float n_other = lane_xor1(nv);
             const float s_other = lane_xor1(sv);
             if (j < n && k < K) {
-                SHT_GLOBAL float2 *dst = reinterpret_cast<SHT_GLOBAL float2 *>(fbuf + ep_row[rb] * (long long)K + k);
+                SHT_GLOBAL float2 *dst = reinterpret_cast<SHT_GLOBAL float2 *>(
+                    fbuf + *reinterpret_cast<const long long *>(slot) * (long long)K + k);
                 if (reim == 0) {
                     gstore(dst, make_float2(nv, n_other));
                 } else {
