@@ -181,7 +181,7 @@ int sht_batch_launch(sht_batch *batch, void *stream);
 int sht_batch_uses_graph(const sht_batch *batch);
 int sht_batch_destroy(sht_batch *batch);
 
-/* Learned column correction (north-star addition; no reference counterpart): an MLP
+/* Learned column correction (an addition of this library; no reference counterpart): an MLP
  * 32 -> 128 -> 128 -> 32 with GELU per grid column (8 levels x {u, v, T, ln ps}), BF16
  * weights with FP32 master copy and FP32 accumulation, trained with Adam on the mean
  * squared error.  x, t, y, grad and loss are device pointers; x, t, y are B x 32 row-major.

@@ -1,6 +1,6 @@
 """CPU restatement of the learned column correction MLP (forward, backward, Adam).
 
-TEST INFRASTRUCTURE ONLY: the checker for the engine's MLP kernels; only ``tests/``,
+TEST INFRASTRUCTURE ONLY: the CPU reference for the engine's MLP kernels; only ``tests/``,
 ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import it.
 
 The network has no counterpart in the SpeedyWeather reference: it is the learned per-column

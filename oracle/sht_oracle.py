@@ -1,6 +1,6 @@
 """FP64 CPU restatement of SpeedyTransforms' spherical harmonic transform.
 
-TEST INFRASTRUCTURE ONLY. This module is the checker the HIP engine is compared
+TEST INFRASTRUCTURE ONLY. This module is the CPU reference the HIP engine is compared
 against; only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 ``bench.py`` may import it. The product (``speedyweather.jl_amd``) never imports or
 calls anything under ``oracle/``.

@@ -11,7 +11,7 @@ where X_coarse starts from trunc(X(t)), i.e. the spectrally truncated fine-run t
 minus the coarse tendency, on the coarse grid per column as tendencies of (u, v, T) on the
 8 levels and of ln ps.  The input is the coarse column at t (u, v, T on the 8 levels, ln ps
 repeated on each level).  A 32 -> 128 -> 128 -> 32 GELU MLP on the engine's kernels learns
-it; the plain spectral tendency stays as the fallback and the judge: scores compare the
+it; the plain spectral tendency stays as the fallback and the yardstick: scores compare the
 coarse model with and without the correction against the truncated fine run, on runs not
 used for training.
 
