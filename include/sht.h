@@ -101,7 +101,8 @@ int sht_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *fie
  * (SpeedyTransforms/src/matrix_transform.jl:56-260; the GPU default for truncations up to 64,
  * SpeedyWeather/src/dynamics/spectral_grid.jl:343-349).  Builds a regular plan plus the two
  * dense transform matrices (FP64-computed, stored FP32); sht_synthesis and sht_analysis then
- * run as one FP32 GEMM each (rocBLAS), every other entry point as on a regular plan.
+ * run as one FP32 GEMM each (a split-K GEMM of this library; the plan's workspace makes
+ * concurrent dense calls on one plan race), every other entry point as on a regular plan.
  * SHT_ERR_UNSUPPORTED if the matrices would exceed 4 GiB. */
 int sht_plan_create_matrix(const sht_desc *desc, sht_plan **plan);
 

@@ -15,8 +15,6 @@ HIPCC_FLAGS = [
     # packed FP32 instructions stay disabled in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
-    # plain FP32 GEMMs of the dense matrix transform
-    "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib",
 ]
 
 

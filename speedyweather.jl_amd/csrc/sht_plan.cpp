@@ -7,7 +7,6 @@
 // Legendre polynomials (spectral_transform.jl:167-172) computed here in FP64 and stored
 // in the order the gfx950 kernels read them.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -306,7 +305,7 @@ struct sht_plan {
     float *mat_syn = nullptr;      // npoints x 2 nharm: columns (Re B, -Im B) of field = Re(B a)
     float *mat_ana = nullptr;      // 2 nharm x npoints: rows (Re F, Im F) of a = F f
     float *pt_coslat_inv = nullptr;  // 1/cos(lat) per grid point (unscale_coslat)
-    rocblas_handle blas = nullptr;
+    float *dense_work = nullptr;     // split-K partial sums of the dense GEMMs
 };
 
 namespace {
@@ -350,7 +349,6 @@ struct DeviceGuard {
 
 void free_plan(sht_plan *p) {
     if (!p) return;
-    if (p->blas) (void)rocblas_destroy_handle(p->blas);
     for (hipEvent_t e : p->prof.ev) (void)hipEventDestroy(e);
     for (void *a : p->allocs) (void)hipFree(a);
     delete p;
@@ -841,11 +839,15 @@ static int build_matrices(sht_plan *pl) {
     pl->mat_ana = const_cast<float *>(d);
     if ((rc = dev_upload(pl->allocs, ci, &d))) return rc;
     pl->pt_coslat_inv = const_cast<float *>(d);
-    if (rocblas_create_handle(&pl->blas) != rocblas_status_success) {
-        pl->blas = nullptr;
-        return fail(SHT_ERR_HIP, "rocblas_create_handle failed");
+    // workspace of the split-K GEMMs: the largest over both directions and every K
+    long long work = 1;
+    for (int K = 1; K <= pl->d.nlayers; ++K) {
+        work = std::max(work, (long long)sht::dense_split(np, 2 * nh, K).s * K * np);
+        work = std::max(work, (long long)sht::dense_split(2 * nh, np, K).s * K * 2 * nh);
     }
-    pl->device_bytes += (long long)bytes + np * 4;
+    if ((rc = dev_upload(pl->allocs, std::vector<float>((size_t)work, 0.f), &d))) return rc;
+    pl->dense_work = const_cast<float *>(d);
+    pl->device_bytes += (long long)bytes + np * 4 + work * 4;
     pl->matrix = true;
     return SHT_OK;
 }
@@ -934,18 +936,11 @@ static int run_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, vo
     if (ld_grid < plan->g.npoints) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_grid < npoints");
     DeviceGuard guard(plan->d.device);
     const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
-    if (plan->matrix && adj == 0 && !accumulate) {  // field = B_int a_int, one GEMM
-        const float one = 1.f, zero = 0.f;
-        if (rocblas_set_stream(plan->blas, reinterpret_cast<hipStream_t>(stream)) != rocblas_status_success ||
-            rocblas_sgemm(plan->blas, rocblas_operation_none, rocblas_operation_none, (int)plan->g.npoints, K,
-                          (int)(2 * plan->g.nharm), &one, plan->mat_syn, (int)plan->g.npoints,
-                          static_cast<const float *>(coeffs), (int)(2 * ld_spec), &zero, static_cast<float *>(field),
-                          (int)ld_grid) != rocblas_status_success)
-            return fail(SHT_ERR_HIP, "rocblas_sgemm (dense synthesis) failed");
-        if (unscale_coslat) {
-            rc = sht::launch_scale_points(plan->pt_coslat_inv, field, plan->g.npoints, ld_grid, K, stream);
-            if (rc) return fail(SHT_ERR_HIP, std::string("unscale_coslat launch: ") + hipGetErrorString((hipError_t)rc));
-        }
+    if (plan->matrix && adj == 0 && !accumulate) {  // field = B_int a_int, one GEMM (rows / cos(lat))
+        rc = sht::launch_dense_gemm(plan->mat_syn, plan->g.npoints, 2 * plan->g.nharm,
+                                    static_cast<const float *>(coeffs), 2 * ld_spec, K, static_cast<float *>(field),
+                                    ld_grid, unscale_coslat ? plan->pt_coslat_inv : nullptr, plan->dense_work, stream);
+        if (rc) return fail(SHT_ERR_HIP, std::string("dense synthesis launch: ") + hipGetErrorString((hipError_t)rc));
         return SHT_OK;
     }
     const CallArgs c{0, K, unscale_coslat ? 1 : 0, plan->fft_pair, coeffs, field, ld_spec, ld_grid, plan->fbuf,
@@ -978,13 +973,10 @@ static int run_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void
     DeviceGuard guard(plan->d.device);
     const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
     if (plan->matrix && adj == 0 && !accumulate) {  // a_int = F_int f, one GEMM
-        const float one = 1.f, zero = 0.f;
-        if (rocblas_set_stream(plan->blas, reinterpret_cast<hipStream_t>(stream)) != rocblas_status_success ||
-            rocblas_sgemm(plan->blas, rocblas_operation_none, rocblas_operation_none, (int)(2 * plan->g.nharm), K,
-                          (int)plan->g.npoints, &one, plan->mat_ana, (int)(2 * plan->g.nharm),
-                          static_cast<const float *>(field), (int)ld_grid, &zero, static_cast<float *>(coeffs),
-                          (int)(2 * ld_spec)) != rocblas_status_success)
-            return fail(SHT_ERR_HIP, "rocblas_sgemm (dense analysis) failed");
+        rc = sht::launch_dense_gemm(plan->mat_ana, 2 * plan->g.nharm, plan->g.npoints,
+                                    static_cast<const float *>(field), ld_grid, K, static_cast<float *>(coeffs),
+                                    2 * ld_spec, nullptr, plan->dense_work, stream);
+        if (rc) return fail(SHT_ERR_HIP, std::string("dense analysis launch: ") + hipGetErrorString((hipError_t)rc));
         return SHT_OK;
     }
     const CallArgs c{0, K, unscale_coslat ? 1 : 0, 0, field, coeffs, ld_grid, ld_spec, plan->fbuf, adj, accumulate ? 1 : 0};

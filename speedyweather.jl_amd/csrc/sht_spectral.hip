@@ -1,4 +1,7 @@
-// Spectral-space operators that feed the transform (vector path of the model step).
+// Spectral-space operators that feed the transform (vector path of the model step), and
+// the dense transform's GEMM.
+#include <algorithm>
+
 #include "sht_device.h"
 
 namespace sht {
@@ -136,6 +139,147 @@ int launch_scale_points(const float *scale, void *field, long long npoints, long
     if (npoints < 1 || K < 1 || K > 65535) return (int)hipErrorInvalidValue;
     const dim3 grid((unsigned)((npoints + 255) / 256), K);
     scale_points_kernel<<<grid, 256, 0, as_stream(stream)>>>(scale, static_cast<float *>(field), npoints, ld);
+    return (int)hipGetLastError();
+}
+
+// ============================================================ dense transform GEMM
+// C (M x K, column stride ldc) = A (M x N, column-major, column stride M) B (N x K, column
+// stride ldb), then optionally C[r, k] *= row_scale[r].  The dense transform's products
+// have few columns (K layers) and a large A that must be read once, so this is a split-K
+// skinny GEMM bound by reading A: workgroup (row block, split, column group) reads rows
+// 1024 b .. + 1023 (thread t: rows t + 256 i, i < 4, so each load instruction reads 1 KiB
+// of a column) of the columns of its split, with the split's rows of B (KB columns) staged
+// in LDS and read as broadcasts; partial sums go to W[s][k][r], which a second kernel adds
+// up in a fixed order (no atomics, deterministic).
+
+constexpr int kDenseRows = 1024;  // rows per workgroup (4 per thread)
+constexpr int kDenseMaxNS = 256;  // columns of A per split at most (LDS for B)
+
+// U columns of A per register set; two sets alternate, so the next set's loads are in
+// flight while the current set's products run (loads past the split re-read its last
+// column and are not used).
+template <int KB>
+__global__ __launch_bounds__(256) void dense_partial_kernel(const float *__restrict__ A_, long long M, long long N,
+                                                            const float *__restrict__ B_, long long ldb, int K,
+                                                            int ns, float *__restrict__ W_) {
+    constexpr int U = KB == 8 ? 8 : 4;
+    __shared__ float Bs[kDenseMaxNS][KB];
+    const int tid = threadIdx.x;
+    const long long r0 = (long long)blockIdx.x * kDenseRows + tid;
+    const int sp = blockIdx.y, k0 = blockIdx.z * KB;
+    const long long n0 = (long long)sp * ns;
+    const int nn = (int)min((long long)ns, N - n0);  // >= 1: the host makes every split non-empty
+    const SHT_GLOBAL float *B = as_global(B_);
+    for (int i = tid; i < nn * KB; i += 256) {
+        const int q = i / KB, k = i - q * KB;
+        Bs[q][k] = k0 + k < K ? B[(n0 + q) + (long long)(k0 + k) * ldb] : 0.f;
+    }
+    int rr[4];  // rows past M re-read row M - 1 (unconditional loads); their sums are dropped
+#pragma unroll
+    for (int i = 0; i < 4; ++i) rr[i] = (int)min(r0 + 256 * i, M - 1);
+    float acc[4][KB];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int k = 0; k < KB; ++k) acc[i][k] = 0.f;
+    const SHT_GLOBAL float *A = as_global(A_) + n0 * M;
+    auto fetch = [&](float (&a)[U][4], int q) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const SHT_GLOBAL float *col = A + (long long)min(q + u, nn - 1) * M;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[u][i] = col[rr[i]];
+        }
+    };
+    auto products = [&](const float (&a)[U][4], int q) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (q + u >= nn) break;  // uniform
+#pragma unroll
+            for (int k = 0; k < KB; ++k) {
+                const float b = Bs[q + u][k];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc[i][k] = fmaf(a[u][i], b, acc[i][k]);
+            }
+        }
+    };
+    float a0[U][4], a1[U][4];
+    fetch(a0, 0);
+    __syncthreads();  // B staged
+    int q = 0;
+    for (; q + U < nn; q += 2 * U) {  // straight-line body: each set's wait covers its own loads
+        fetch(a1, q + U);
+        products(a0, q);
+        fetch(a0, q + 2 * U);
+        products(a1, q + U);
+    }
+    if (q < nn) products(a0, q);
+    SHT_GLOBAL float *W = as_global(W_);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const long long r = r0 + 256 * i;
+        if (r >= M) continue;
+#pragma unroll
+        for (int k = 0; k < KB; ++k)
+            if (k0 + k < K) W[((long long)sp * K + k0 + k) * M + r] = acc[i][k];
+    }
+}
+
+// Sum of the S partials of 32 rows of one column k: 8 thread groups each add every 8th
+// split (independent loads in flight), then group 0 adds the 8 group sums in order.
+constexpr int kReduceRows = 32, kReduceGroups = 8;
+__global__ __launch_bounds__(256) void dense_reduce_kernel(const float *__restrict__ W_, long long M, int K, int S,
+                                                           const float *__restrict__ row_scale,
+                                                           float *__restrict__ C_, long long ldc) {
+    __shared__ float part[kReduceGroups][kReduceRows];
+    const int row = threadIdx.x % kReduceRows, grp = threadIdx.x / kReduceRows;
+    const long long r = (long long)blockIdx.x * kReduceRows + row;
+    const int k = blockIdx.y;
+    const long long rc = r < M ? r : M - 1;  // rows past M read row M - 1 and are not stored
+    const SHT_GLOBAL float *W = as_global(W_) + (long long)k * M + rc;
+    const long long step = (long long)K * M;
+    float sum = 0.f;
+#pragma unroll 4
+    for (int sp = grp; sp < S; sp += kReduceGroups) sum += W[(long long)sp * step];
+    part[grp][row] = sum;
+    __syncthreads();
+    if (grp != 0 || r >= M) return;
+    float t = part[0][row];
+#pragma unroll
+    for (int g = 1; g < kReduceGroups; ++g) t += part[g][row];
+    if (row_scale) t *= as_global(row_scale)[r];
+    as_global(C_)[r + (long long)k * ldc] = t;
+}
+
+DenseSplit dense_split(long long M, long long N, int K) {
+    DenseSplit d;
+    d.kb = K <= 8 ? 8 : 16;
+    d.rb = (int)((M + kDenseRows - 1) / kDenseRows);
+    d.cg = (K + d.kb - 1) / d.kb;
+    long long s = (512 + (long long)d.rb * d.cg - 1) / ((long long)d.rb * d.cg);  // about 512 workgroups
+    s = std::min(s, (N + 31) / 32);                                               // at least 32 columns each
+    s = std::max(s, (N + kDenseMaxNS - 1) / kDenseMaxNS);                         // at most kDenseMaxNS
+    s = std::max(s, 1LL);
+    d.ns = (int)((N + s - 1) / s);
+    d.s = (int)((N + d.ns - 1) / d.ns);  // every split non-empty
+    return d;
+}
+
+int launch_dense_gemm(const float *A, long long M, long long N, const float *B, long long ldb, int K, float *C,
+                      long long ldc, const float *row_scale, float *work, void *stream) {
+    if (M < 1 || N < 1 || K < 1 || K > 65535 || ldb < N || ldc < M) return (int)hipErrorInvalidValue;
+    const DenseSplit d = dense_split(M, N, K);
+    if (d.s > 65535 || d.cg > 65535) return (int)hipErrorInvalidValue;
+    const hipStream_t st = as_stream(stream);
+    const dim3 grid(d.rb, d.s, d.cg);
+    if (d.kb == 8)
+        dense_partial_kernel<8><<<grid, 256, 0, st>>>(A, M, N, B, ldb, K, d.ns, work);
+    else
+        dense_partial_kernel<16><<<grid, 256, 0, st>>>(A, M, N, B, ldb, K, d.ns, work);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+    dense_reduce_kernel<<<dim3((unsigned)((M + kReduceRows - 1) / kReduceRows), K), 256, 0, st>>>(work, M, K, d.s,
+                                                                                                row_scale, C, ldc);
     return (int)hipGetLastError();
 }
 

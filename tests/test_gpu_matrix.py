@@ -53,6 +53,27 @@ def test_matrix_transform_vs_oracle(sw, gpu, name):
     S.close()
 
 
+@pytest.mark.parametrize("K", [20, 9, 1])
+def test_matrix_transform_column_groups(sw, gpu, K):
+    """Column counts past one group of the split-K GEMM (8 or 16 columns per workgroup),
+    and a single column, in both directions, on a plan holding 20 layers."""
+    kind, n, lmax, mmax = CASES["T31"]
+    o = oracle(kind, n, lmax, mmax)
+    M = make(sw, "T31", 20, sw.MatrixSpectralTransform)
+    print(f"matrix transform T31 K={K}", flush=True)
+    a = sw.seeded_spectrum(M.spectrum, K, seed=23)
+    f_ref = o.synthesis(a)
+    c = to_dev(sw, a, M.spectrum)
+    f = sw.transform(c, M)
+    fu = sw.transform(c, M, unscale_coslat=True)
+    b = sw.transform(sw.Field(torch.from_numpy(f_ref.astype(np.float32)).cuda(), M.grid), M)
+    torch.cuda.synchronize()
+    assert rel(f.data.cpu().numpy(), f_ref) < 1e-5
+    assert rel(fu.data.cpu().numpy(), o.synthesis(a, unscale_coslat=True)) < 1e-5
+    assert rel(b.data.cpu().numpy(), o.analysis(f_ref)) < 1e-5
+    M.close()
+
+
 def test_matrix_transform_size_guard(sw, gpu):
     with pytest.raises(sw.SHTError):
         sw.MatrixSpectralTransform(sw.Spectrum(257, 256), sw.OctahedralGaussianGrid(192), nlayers=1)
