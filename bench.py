@@ -150,6 +150,36 @@ def bench_dense(sw, torch, steps, warmup):
     return out
 
 
+def bench_fused_laplace(sw, torch, steps, warmup):
+    """T255 K=64 (FP16x3): the grid -> spectral transform fused with ∇² added into the output
+    with flipped sign (the Bernoulli potential into the divergence tendency) against the same
+    as transform_ then laplace_; and ∇⁻² fused into the spectral -> grid transform (the
+    streamfunction from vorticity) against laplace_ then transform_."""
+    sp, grid, K = sw.Spectrum(257, 256), sw.OctahedralGaussianGrid(192), 64
+    S = sw.SpectralTransform(sp, grid, nlayers=K, precision="fp16x3")
+    S.autotune(K)
+    c = sw.LowerTriangularArray(torch.from_numpy(sw.seeded_spectrum(sp, K, 13)).to(torch.complex64).cuda(), sp)
+    f = sw.transform(c, S)
+    out = sw.LowerTriangularArray(torch.zeros_like(c.data), sp)
+    tmp = sw.LowerTriangularArray(torch.zeros_like(c.data), sp)
+    g = sw.Field(torch.empty_like(f.data), grid)
+    res = {
+        "analysis_laplace_fused_ms": time_loop(torch, lambda: sw.transform_laplace_(out, f, S, flipsign=True, add=True),
+                                               steps, warmup) * 1e3,
+        "analysis_then_laplace_ms": time_loop(torch, lambda: (sw.transform_(tmp, f, S),
+                                                              sw.laplace_(out, tmp, S, flipsign=True, add=True)),
+                                              steps, warmup) * 1e3,
+        "inverse_laplace_synthesis_fused_ms": time_loop(torch, lambda: sw.transform_laplace_(g, c, S, inverse=True),
+                                                        steps, warmup) * 1e3,
+        "inverse_laplace_then_synthesis_ms": time_loop(torch, lambda: (sw.laplace_(tmp, c, S, inverse=True),
+                                                                       sw.transform_(g, tmp, S)),
+                                                       steps, warmup) * 1e3,
+        "workload": "T255 octahedral nlat_half=192, K=64, FP16x3",
+    }
+    S.close()
+    return res
+
+
 def bench_c4(sw, torch, steps, warmup):
     """64 independent round trips, truncations from {T31,T63,T127,T255} (octahedral), layers
     U{1..16} x fields U{1..4}, seed 1234, FP32: one batched launch per stage from a hipGraph,
@@ -302,6 +332,7 @@ def main():
         extra["c2_t127_vector_path"] = bench_c2(sw, torch, args.steps, args.warmup)
         extra["c4_uneven_batch64"] = bench_c4(sw, torch, args.steps, args.warmup)
         extra["dense_matrix_transform"] = bench_dense(sw, torch, args.steps, args.warmup)
+        extra["fused_laplace"] = bench_fused_laplace(sw, torch, args.steps, args.warmup)
         extra["c5_learned_correction"] = bench_c5(sw, seeds=args.c5_seeds)
 
     # accuracy against the FP64 oracle of the same algorithm (per field and degree band)

@@ -148,6 +148,21 @@ int sht_laplace(sht_plan *plan, const void *alms, void *out, int64_t ld, int32_t
 int sht_gradient(sht_plan *plan, const void *p, void *dpdx, void *dpdy, int64_t ld, int32_t K, double radius,
                  void *stream);
 
+/* Transforms fused with the spectral Laplacian: one call replaces a transform plus a
+ * ∇²! / ∇⁻²! pass (spectral_gradients.jl:385-466) over the spectra, with the Laplacian
+ * applied inside the Legendre stage.  No single reference call: they compose
+ * transform!(field, ∇²(alms), S) (e.g. the streamfunction from vorticity through ∇⁻²) and
+ * ∇²!(out, transform!(alms, field, S); add, flipsign) (the Bernoulli potential's ∇² into
+ * the divergence tendency, SpeedyWeather/src/dynamics/tendencies.jl:1123-1212).  inverse,
+ * flipsign, add and radius as in sht_laplace (add accumulates into coeffs, then the radius
+ * scaling applies); layouts, strides and streams as in sht_synthesis / sht_analysis.  On a
+ * dense (matrix) plan they run the regular transform kernels. */
+int sht_synthesis_laplace(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *field, int64_t ld_grid,
+                          int32_t K, int32_t inverse, int32_t flipsign, double radius, int32_t unscale_coslat,
+                          void *stream);
+int sht_analysis_laplace(sht_plan *plan, const void *field, int64_t ld_grid, void *coeffs, int64_t ld_spec, int32_t K,
+                         int32_t inverse, int32_t flipsign, int32_t add, double radius, void *stream);
+
 /* Uneven batches: many independent transforms (any mix of plans, i.e. truncations and
  * grids, layer counts and fields) run as ONE launch per stage, their work tiles merged
  * and ordered heaviest first, replayed from an explicitly built hipGraph.  No reference

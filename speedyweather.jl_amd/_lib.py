@@ -28,6 +28,7 @@ EXPORTS = (
     "sht_mlp_get_params", "sht_mlp_forward", "sht_mlp_gradient", "sht_mlp_train_step", "sht_mlp_destroy",
     "sht_plan_autotune", "sht_plan_tuning", "sht_plan_set_tuning", "sht_spec2grid_pullback",
     "sht_grid2spec_pullback", "sht_divergence", "sht_laplace", "sht_gradient", "sht_plan_create_matrix",
+    "sht_synthesis_laplace", "sht_analysis_laplace",
 )
 
 
@@ -83,6 +84,8 @@ def lib() -> C.CDLL:
         L.sht_grid2spec_pullback.argtypes = [vp, vp, i64, vp, i64, i32, i32, vp]
         L.sht_divergence.argtypes = [vp, vp, vp, vp, i64, i32, i32, i32, i32, C.c_double, vp]
         L.sht_laplace.argtypes = [vp, vp, vp, i64, i32, i32, i32, i32, C.c_double, vp]
+        L.sht_synthesis_laplace.argtypes = [vp, vp, i64, vp, i64, i32, i32, i32, C.c_double, i32, vp]
+        L.sht_analysis_laplace.argtypes = [vp, vp, i64, vp, i64, i32, i32, i32, i32, C.c_double, vp]
         L.sht_gradient.argtypes = [vp, vp, vp, vp, i64, i32, C.c_double, vp]
         L.sht_uv_from_vordiv.argtypes = [vp, vp, vp, vp, vp, i64, i32, C.c_double, vp]
         L.sht_gauss_legendre.argtypes = [i32, vp, vp]

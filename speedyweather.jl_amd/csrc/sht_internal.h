@@ -74,6 +74,14 @@ struct CallArgs {
     // into its output instead of overwriting it.
     int adj;
     int accumulate;
+    // Fused spectral Laplacian (spectral_gradients.jl:385-466): lap 0 none, 1 ∇² (degree
+    // factor e_l = -l(l+1)), 2 ∇⁻² (1/e_l, 0 at l = 0).  The Legendre synthesis multiplies
+    // its input degrees by lap_sign e_l lap_scale; the Legendre analysis multiplies its
+    // output by lap_sign e_l, adds the old output if accumulate, then scales by lap_scale
+    // (the reference's order).  Zero-initialised calls (lap = 0) are plain transforms.
+    int lap;
+    float lap_sign;
+    float lap_scale;
 };
 
 enum Stage { kLegendreSynthesis = 0, kFFTSynthesis = 1, kFFTAnalysis = 2, kLegendreAnalysis = 3 };

@@ -156,6 +156,12 @@ __device__ inline void mma_chunk1(const unsigned char *sm, int buf, int p, int r
 // Per-ring epilogue factors of ring rb < 64 of a synthesis block: 16 bytes in the padding of
 // staging row rb (parity 0, buffer 0; the FP32 row, or the lo row of the split modes), which
 // the staging and the products never touch: the Fourier row offset, then the row scale.
+// Degree factor of a fused Laplacian: lap 1: e_l = -l(l+1); lap 2: 1 / e_l, 0 at l = 0.
+__device__ __forceinline__ float lap_factor(int lap, int l) {
+    const float e = -(float)l * (float)(l + 1);
+    return lap == 1 ? e : (l == 0 ? 0.f : 1.f / e);
+}
+
 template <int MODE>
 __device__ inline unsigned char *ep_slot(unsigned char *sm, int rb) {
     if constexpr (MODE == kFP32)
@@ -257,6 +263,14 @@ __global__ __launch_bounds__(256, 4) void legendre_synthesis_kernel(const DevPla
                 const bool ok = kok && 64 * c + 8 * g + 2 * i + p < L;
                 re[i] = ok ? cur.v[2 * i + p].x : 0.f;
                 im[i] = ok ? cur.v[2 * i + p].y : 0.f;
+            }
+            if (C.lap) {  // fused Laplacian of the input degrees (uniform)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float f = C.lap_sign * C.lap_scale * lap_factor(C.lap, m + 64 * c + 8 * g + 2 * i + p);
+                    re[i] *= f;
+                    im[i] *= f;
+                }
             }
             stage4<MODE>(sm, buf, p, kk, g, re, im);
         }
@@ -487,17 +501,35 @@ __global__ __launch_bounds__(256, 3) void legendre_analysis_kernel(const DevPlan
             }
         }
     __syncthreads();
+    constexpr int NIT = kColsComplex * DPW / 256;
+    // accumulate: every old output of this thread is read before the first store (a store
+    // between them would keep the compiler from issuing the later reads early)
+    float2 old[NIT];
+    if (C.accumulate) {  // uniform
 #pragma unroll
-    for (int it = 0; it < kColsComplex * DPW / 256; ++it) {
+        for (int it = 0; it < NIT; ++it) {
+            const int idx = it * 256 + tid;
+            const int kc = idx / DPW, o = idx % DPW;
+            const int og = DPW * T.blk + o, kq = kbase + kc;
+            old[it] = make_float2(0.f, 0.f);
+            if (og < L && kq < K) old[it] = gload(coeffs + (long long)kq * ld_spec + off + og);
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
         const int idx = it * 256 + tid;
         const int kc = idx / DPW, o = idx % DPW;
         const int og = DPW * T.blk + o, kq = kbase + kc;
         if (og < L && kq < K) {
             SHT_GLOBAL float2 *dst = coeffs + (long long)kq * ld_spec + off + og;
             float2 v = Out[kc * (DPW + 1) + o];
-            if (C.accumulate) {  // pullback: add into the caller's cotangent
-                const float2 u = gload(dst);
-                v = make_float2(v.x + u.x, v.y + u.y);
+            if (C.lap) {  // fused Laplacian: sign e_l v (+ old), then the radius scaling
+                const float f = C.lap_sign * lap_factor(C.lap, m + og);
+                v = make_float2(f * v.x, f * v.y);
+                if (C.accumulate) v = make_float2(v.x + old[it].x, v.y + old[it].y);
+                v = make_float2(C.lap_scale * v.x, C.lap_scale * v.y);
+            } else if (C.accumulate) {  // pullback: add into the caller's cotangent
+                v = make_float2(v.x + old[it].x, v.y + old[it].y);
             }
             gstore(dst, v);
         }

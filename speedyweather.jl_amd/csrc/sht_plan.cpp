@@ -926,17 +926,23 @@ static int check_call(sht_plan *plan, const void *a, const void *b, int K) {
     return SHT_OK;
 }
 
+// A fused spectral Laplacian of a transform call (CallArgs::lap and its factors).
+struct LapOp {
+    int lap = 0;  // 0 none, 1 ∇², 2 ∇⁻²
+    float sign = 1.f, scale = 1.f;
+};
+
 // Spectral -> grid stages (Legendre synthesis, FFT synthesis); adj = 2 and accumulate turn
-// them into the pullback of the analysis.
+// them into the pullback of the analysis; op fuses a Laplacian of the input.
 static int run_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *field, int64_t ld_grid, int32_t K,
-                         int32_t unscale_coslat, int adj, int accumulate, void *stream) {
+                         int32_t unscale_coslat, int adj, int accumulate, void *stream, LapOp op = LapOp()) {
     int rc = check_call(plan, coeffs, field, K);
     if (rc) return rc;
     if (ld_spec < plan->g.nharm) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_spec < nharm");
     if (ld_grid < plan->g.npoints) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_grid < npoints");
     DeviceGuard guard(plan->d.device);
     const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
-    if (plan->matrix && adj == 0 && !accumulate) {  // field = B_int a_int, one GEMM (rows / cos(lat))
+    if (plan->matrix && adj == 0 && !accumulate && op.lap == 0) {  // field = B_int a_int, one GEMM (rows / cos(lat))
         rc = sht::launch_dense_gemm(plan->mat_syn, plan->g.npoints, 2 * plan->g.nharm,
                                     static_cast<const float *>(coeffs), 2 * ld_spec, K, static_cast<float *>(field),
                                     ld_grid, unscale_coslat ? plan->pt_coslat_inv : nullptr, plan->dense_work, stream);
@@ -944,7 +950,7 @@ static int run_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, vo
         return SHT_OK;
     }
     const CallArgs c{0, K, unscale_coslat ? 1 : 0, plan->fft_pair, coeffs, field, ld_spec, ld_grid, plan->fbuf,
-                     adj, accumulate ? 1 : 0};
+                     adj, accumulate ? 1 : 0, op.lap, op.sign, op.scale};
     hipError_t e;
     {
         ProfScope ps(plan, SHT_KERNEL_LEGENDRE_SYNTHESIS, stream);
@@ -963,23 +969,24 @@ static int run_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, vo
 }
 
 // Grid -> spectral stages (FFT analysis, Legendre analysis); adj = 1 and accumulate turn
-// them into the pullback of the synthesis.
+// them into the pullback of the synthesis; op fuses a Laplacian of the output.
 static int run_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void *coeffs, int64_t ld_spec, int32_t K,
-                        int32_t unscale_coslat, int adj, int accumulate, void *stream) {
+                        int32_t unscale_coslat, int adj, int accumulate, void *stream, LapOp op = LapOp()) {
     int rc = check_call(plan, field, coeffs, K);
     if (rc) return rc;
     if (ld_spec < plan->g.nharm) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_spec < nharm");
     if (ld_grid < plan->g.npoints) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_grid < npoints");
     DeviceGuard guard(plan->d.device);
     const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
-    if (plan->matrix && adj == 0 && !accumulate) {  // a_int = F_int f, one GEMM
+    if (plan->matrix && adj == 0 && !accumulate && op.lap == 0) {  // a_int = F_int f, one GEMM
         rc = sht::launch_dense_gemm(plan->mat_ana, 2 * plan->g.nharm, plan->g.npoints,
                                     static_cast<const float *>(field), ld_grid, K, static_cast<float *>(coeffs),
                                     2 * ld_spec, nullptr, plan->dense_work, stream);
         if (rc) return fail(SHT_ERR_HIP, std::string("dense analysis launch: ") + hipGetErrorString((hipError_t)rc));
         return SHT_OK;
     }
-    const CallArgs c{0, K, unscale_coslat ? 1 : 0, 0, field, coeffs, ld_grid, ld_spec, plan->fbuf, adj, accumulate ? 1 : 0};
+    const CallArgs c{0,           K,          unscale_coslat ? 1 : 0, 0,       field,   coeffs, ld_grid, ld_spec,
+                     plan->fbuf,  adj,        accumulate ? 1 : 0,     op.lap, op.sign, op.scale};
     hipError_t e;
     {
         ProfScope ps(plan, SHT_KERNEL_FFT_ANALYSIS, stream);
@@ -1015,6 +1022,33 @@ int sht_spec2grid_pullback(sht_plan *plan, const void *field_bar, int64_t ld_gri
 int sht_grid2spec_pullback(sht_plan *plan, const void *coeffs_bar, int64_t ld_spec, void *field_bar, int64_t ld_grid,
                            int32_t K, int32_t accumulate, void *stream) {
     return run_synthesis(plan, coeffs_bar, ld_spec, field_bar, ld_grid, K, 0, 2, accumulate, stream);
+}
+
+// The Laplacian's factors on a sphere of the given radius (as sht_laplace: 1/R² for ∇², R²
+// for ∇⁻²), or an error status.
+static int lap_op(int32_t inverse, int32_t flipsign, double radius, LapOp &op) {
+    if (!(radius > 0)) return fail(SHT_ERR_ARG, "radius must be positive");
+    const double r2 = radius * radius;
+    op.lap = inverse ? 2 : 1;
+    op.sign = flipsign ? -1.f : 1.f;
+    op.scale = (float)(inverse ? r2 : 1.0 / r2);
+    return SHT_OK;
+}
+
+int sht_synthesis_laplace(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *field, int64_t ld_grid, int32_t K,
+                          int32_t inverse, int32_t flipsign, double radius, int32_t unscale_coslat, void *stream) {
+    LapOp op;
+    const int rc = lap_op(inverse, flipsign, radius, op);
+    if (rc) return rc;
+    return run_synthesis(plan, coeffs, ld_spec, field, ld_grid, K, unscale_coslat, 0, 0, stream, op);
+}
+
+int sht_analysis_laplace(sht_plan *plan, const void *field, int64_t ld_grid, void *coeffs, int64_t ld_spec, int32_t K,
+                         int32_t inverse, int32_t flipsign, int32_t add, double radius, void *stream) {
+    LapOp op;
+    const int rc = lap_op(inverse, flipsign, radius, op);
+    if (rc) return rc;
+    return run_analysis(plan, field, ld_grid, coeffs, ld_spec, K, 0, 0, add ? 1 : 0, stream, op);
 }
 
 int sht_uv_from_vordiv(sht_plan *plan, const void *vor, const void *div, void *U, void *V, int64_t ld, int32_t K,

@@ -273,6 +273,46 @@ def inverse_laplace_(out: LowerTriangularArray, alms: LowerTriangularArray, S: S
     return laplace_(out, alms, S, add=add, flipsign=flipsign, inverse=True, radius=radius)
 
 
+def transform_laplace_(out, inp, S: SpectralTransform, inverse: bool = False, flipsign: bool = False,
+                       add: bool = False, radius: float = 1.0, unscale_coslat: bool = False):
+    """A transform fused with the spectral Laplacian (one call, the Laplacian applied in the
+    Legendre stage; the reference composes transform! and ∇²! / ∇⁻²!,
+    spectral_gradients.jl:385-466):
+    Field <- LowerTriangularArray: out = transform(∇² inp) (or ∇⁻² with inverse=True, negated
+    with flipsign, optionally divided by cos(lat));
+    LowerTriangularArray <- Field: out = ∇² transform(inp), with add and flipsign as in
+    laplace_ (add accumulates into out, then the radius scaling applies).  Returns out."""
+    if isinstance(out, Field) and isinstance(inp, LowerTriangularArray):
+        if add:
+            raise TypeError("add applies to grid->spectral transforms only")
+        for x in (out, inp):
+            if not S.ismatching(x):
+                raise _mismatch(S, x)
+        if out.nlayers != inp.nlayers:
+            raise DimensionMismatch(f"field K={out.nlayers} and coeffs K={inp.nlayers} differ")
+        c2, K, ld_spec = S._check_tensor(inp.data, S.spectrum.nharm, torch.complex64, "coeffs")
+        f2, _, ld_grid = S._check_tensor(out.data, S.grid.npoints, torch.float32, "field")
+        check(lib().sht_synthesis_laplace(S._plan, c2.data_ptr(), ld_spec, f2.data_ptr(), ld_grid, K, int(inverse),
+                                          int(flipsign), float(radius), int(bool(unscale_coslat)), S._stream()),
+              "sht_synthesis_laplace")
+        return out
+    if isinstance(out, LowerTriangularArray) and isinstance(inp, Field):
+        if unscale_coslat:
+            raise TypeError("unscale_coslat applies to spectral->grid transforms only")
+        for x in (out, inp):
+            if not S.ismatching(x):
+                raise _mismatch(S, x)
+        if out.nlayers != inp.nlayers:
+            raise DimensionMismatch(f"coeffs K={out.nlayers} and field K={inp.nlayers} differ")
+        f2, K, ld_grid = S._check_tensor(inp.data, S.grid.npoints, torch.float32, "field")
+        c2, _, ld_spec = S._check_tensor(out.data, S.spectrum.nharm, torch.complex64, "coeffs")
+        check(lib().sht_analysis_laplace(S._plan, f2.data_ptr(), ld_grid, c2.data_ptr(), ld_spec, K, int(inverse),
+                                         int(flipsign), int(add), float(radius), S._stream()),
+              "sht_analysis_laplace")
+        return out
+    raise TypeError(f"transform_laplace_({type(out).__name__}, {type(inp).__name__}) is not defined")
+
+
 def gradient_(dpdx: LowerTriangularArray, dpdy: LowerTriangularArray, p: LowerTriangularArray, S: SpectralTransform,
               radius: float = 1.0):
     """dpdx = ∂p/∂λ, dpdy = cos(lat) ∂p/∂φ (∇!, spectral_gradients.jl:483-527).  Returns (dpdx, dpdy)."""
