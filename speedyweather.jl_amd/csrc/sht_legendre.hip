@@ -502,18 +502,26 @@ __global__ __launch_bounds__(256, 3) void legendre_analysis_kernel(const DevPlan
         }
     __syncthreads();
     constexpr int NIT = kColsComplex * DPW / 256;
-    // accumulate: every old output of this thread is read before the first store (a store
-    // between them would keep the compiler from issuing the later reads early)
-    float2 old[NIT];
-    if (C.accumulate) {  // uniform
+    if (!C.accumulate && !C.lap) {  // plain transform (uniform)
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
             const int idx = it * 256 + tid;
             const int kc = idx / DPW, o = idx % DPW;
             const int og = DPW * T.blk + o, kq = kbase + kc;
-            old[it] = make_float2(0.f, 0.f);
-            if (og < L && kq < K) old[it] = gload(coeffs + (long long)kq * ld_spec + off + og);
+            if (og < L && kq < K) gstore(coeffs + (long long)kq * ld_spec + off + og, Out[kc * (DPW + 1) + o]);
         }
+        return;
+    }
+    // accumulate: every old output of this thread is read before the first store (a store
+    // between them would keep the compiler from issuing the later reads early)
+    float2 old[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int idx = it * 256 + tid;
+        const int kc = idx / DPW, o = idx % DPW;
+        const int og = DPW * T.blk + o, kq = kbase + kc;
+        old[it] = make_float2(0.f, 0.f);
+        if (C.accumulate && og < L && kq < K) old[it] = gload(coeffs + (long long)kq * ld_spec + off + og);
     }
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
@@ -521,17 +529,15 @@ __global__ __launch_bounds__(256, 3) void legendre_analysis_kernel(const DevPlan
         const int kc = idx / DPW, o = idx % DPW;
         const int og = DPW * T.blk + o, kq = kbase + kc;
         if (og < L && kq < K) {
-            SHT_GLOBAL float2 *dst = coeffs + (long long)kq * ld_spec + off + og;
             float2 v = Out[kc * (DPW + 1) + o];
             if (C.lap) {  // fused Laplacian: sign e_l v (+ old), then the radius scaling
                 const float f = C.lap_sign * lap_factor(C.lap, m + og);
-                v = make_float2(f * v.x, f * v.y);
-                if (C.accumulate) v = make_float2(v.x + old[it].x, v.y + old[it].y);
+                v = make_float2(f * v.x + old[it].x, f * v.y + old[it].y);
                 v = make_float2(C.lap_scale * v.x, C.lap_scale * v.y);
-            } else if (C.accumulate) {  // pullback: add into the caller's cotangent
+            } else {  // pullback: add into the caller's cotangent
                 v = make_float2(v.x + old[it].x, v.y + old[it].y);
             }
-            gstore(dst, v);
+            gstore(coeffs + (long long)kq * ld_spec + off + og, v);
         }
     }
 }
