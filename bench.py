@@ -180,6 +180,28 @@ def bench_fused_laplace(sw, torch, steps, warmup):
     return res
 
 
+def bench_pullbacks(sw, torch, steps, warmup):
+    """T255 K=64 (FP16x3): the adjoint transforms (spec2grid_pullback!, grid2spec_pullback!)
+    accumulating into their outputs as the reference's Enzyme rules do, beside the forward
+    transforms they mirror (ms per call)."""
+    sp, grid, K = sw.Spectrum(257, 256), sw.OctahedralGaussianGrid(192), 64
+    S = sw.SpectralTransform(sp, grid, nlayers=K, precision="fp16x3")
+    S.autotune(K)
+    c = sw.LowerTriangularArray(torch.from_numpy(sw.seeded_spectrum(sp, K, 17)).to(torch.complex64).cuda(), sp)
+    f = sw.transform(c, S)
+    cb = sw.LowerTriangularArray(torch.zeros_like(c.data), sp)
+    fb = sw.Field(torch.zeros_like(f.data), grid)
+    res = {
+        "synthesis_ms": time_loop(torch, lambda: sw.transform_(fb, c, S), steps, warmup) * 1e3,
+        "analysis_ms": time_loop(torch, lambda: sw.transform_(cb, f, S), steps, warmup) * 1e3,
+        "spec2grid_pullback_ms": time_loop(torch, lambda: sw.spec2grid_pullback_(cb, f, S), steps, warmup) * 1e3,
+        "grid2spec_pullback_ms": time_loop(torch, lambda: sw.grid2spec_pullback_(fb, c, S), steps, warmup) * 1e3,
+        "workload": "T255 octahedral nlat_half=192, K=64, FP16x3, pullbacks accumulating",
+    }
+    S.close()
+    return res
+
+
 def bench_c4(sw, torch, steps, warmup):
     """64 independent round trips, truncations from {T31,T63,T127,T255} (octahedral), layers
     U{1..16} x fields U{1..4}, seed 1234, FP32: one batched launch per stage from a hipGraph,
@@ -333,6 +355,7 @@ def main():
         extra["c4_uneven_batch64"] = bench_c4(sw, torch, args.steps, args.warmup)
         extra["dense_matrix_transform"] = bench_dense(sw, torch, args.steps, args.warmup)
         extra["fused_laplace"] = bench_fused_laplace(sw, torch, args.steps, args.warmup)
+        extra["pullbacks"] = bench_pullbacks(sw, torch, args.steps, args.warmup)
         extra["c5_learned_correction"] = bench_c5(sw, seeds=args.c5_seeds)
 
     # accuracy against the FP64 oracle of the same algorithm (per field and degree band)

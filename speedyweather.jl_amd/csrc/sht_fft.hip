@@ -605,8 +605,10 @@ constexpr int kFFTRowPasses = (kFFTMaxRing / 2 + kFFTRowsPerPass) / kFFTRowsPerP
 
 // spectral -> grid, Fourier stage (fourier.jl:215-250): per ring, brfft of g_N and g_S as
 // one complex inverse FFT of Z = g_N + i g_S (Hermitian-extended; Im of the DC and
-// Nyquist bins ignored as FFTW's c2r does; orders above mt are zero).
-template <bool BATCH>
+// Nyquist bins ignored as FFTW's c2r does; orders above mt are zero).  ACC: the pullback
+// that adds into the caller's grid (its own instantiation, so the plain transform keeps
+// its registers).
+template <bool BATCH, bool ACC = false>
 __global__ __launch_bounds__(kFFTThreads, 5) void fft_synthesis_kernel(const DevPlan *__restrict__ plans,
                                                                     CallArgs single,
                                                                     const CallArgs *__restrict__ calls,
@@ -675,15 +677,31 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_synthesis_kernel(const Dev
             SHT_GLOBAL float *fn = field + (long long)(G.k0 + c) * ld + rn;
             SHT_GLOBAL float *fs = field + (long long)(G.k0 + c) * ld + rs;
             const float2 *z = b0 + c * N;
+            if constexpr (ACC) {
+                // pullback: add into the caller's cotangent; the column's old values are all
+                // read before its first store, so the reads are in flight together
+                float on[U], os[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int i = u * kFFTThreads + tid;
-                if (i < N) {
-                    const float2 v = z[i];
-                    if (C.accumulate) {  // pullback: add into the caller's cotangent
-                        fn[(unsigned)i] += v.x;
-                        fs[(unsigned)i] += v.y;
-                    } else {
+                for (int u = 0; u < U; ++u) {
+                    const int i = min(u * kFFTThreads + tid, N - 1);
+                    on[u] = fn[(unsigned)i];
+                    os[u] = fs[(unsigned)i];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int i = u * kFFTThreads + tid;
+                    if (i < N) {
+                        const float2 v = z[i];
+                        fn[(unsigned)i] = on[u] + v.x;
+                        fs[(unsigned)i] = os[u] + v.y;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int i = u * kFFTThreads + tid;
+                    if (i < N) {
+                        const float2 v = z[i];
                         fn[(unsigned)i] = v.x;
                         fs[(unsigned)i] = v.y;
                     }
@@ -769,10 +787,12 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_analysis_kernel(const DevP
 
 // ================================================================== entry points
 
-const void *fft_kernel(int stage, bool batch) {
-    if (stage == kFFTSynthesis)
+const void *fft_kernel(int stage, bool batch, bool accumulate) {
+    if (stage == kFFTSynthesis) {
+        if (accumulate && !batch) return reinterpret_cast<const void *>(&fft_synthesis_kernel<false, true>);
         return batch ? reinterpret_cast<const void *>(&fft_synthesis_kernel<true>)
                      : reinterpret_cast<const void *>(&fft_synthesis_kernel<false>);
+    }
     return batch ? reinterpret_cast<const void *>(&fft_analysis_kernel<true>)
                  : reinterpret_cast<const void *>(&fft_analysis_kernel<false>);
 }

@@ -161,7 +161,9 @@ void set_last_error(const std::string &msg);
 // kernels read their tile from tiles[blockIdx.x], the FFT kernels derive ring and column
 // group from blockIdx.x.  batch = true: tile, call and column group from tiles.
 const void *legendre_kernel(int stage, int mode, bool batch);
-const void *fft_kernel(int stage, bool batch);
+// accumulate: the single-call synthesis that adds into its output (pullbacks); batches
+// never accumulate.
+const void *fft_kernel(int stage, bool batch, bool accumulate = false);
 // Spectral gradient operators (one kernel, op code; flags: kOp* bits).
 enum SpectralOp { kOpDivergence = 0, kOpCurl = 1, kOpLaplace = 2, kOpGradient = 3 };
 constexpr int kOpFlip = 1, kOpAdd = 2, kOpInverse = 4;

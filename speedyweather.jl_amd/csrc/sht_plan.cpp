@@ -740,6 +740,8 @@ int build_plan(sht_plan *pl) {
             for (int st : {sht::kFFTSynthesis, sht::kFFTAnalysis})
                 (void)hipFuncSetAttribute(sht::fft_kernel(st, b), hipFuncAttributeMaxDynamicSharedMemorySize,
                                           P.fft_smem);
+        (void)hipFuncSetAttribute(sht::fft_kernel(sht::kFFTSynthesis, false, true),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, P.fft_smem);
     }
     const size_t fb_bytes = (size_t)g.fb_rows * d.nlayers * 16;
     hipError_t e = hipMalloc(&pl->fbuf, fb_bytes);
@@ -961,7 +963,8 @@ static int run_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, vo
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("legendre synthesis launch: ") + hipGetErrorString(e));
     {
         ProfScope ps(plan, SHT_KERNEL_FFT_SYNTHESIS, stream);
-        e = launch_stage(sht::fft_kernel(sht::kFFTSynthesis, false), dim3(fft_blocks(plan->g.n, K, plan->fft_pair != 0)),
+        e = launch_stage(sht::fft_kernel(sht::kFFTSynthesis, false, accumulate != 0),
+                         dim3(fft_blocks(plan->g.n, K, plan->fft_pair != 0)),
                          plan->P.fft_smem, plan->dP, c, nullptr, nullptr, stream);
     }
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("fft synthesis launch: ") + hipGetErrorString(e));
