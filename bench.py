@@ -37,17 +37,39 @@ def geometry(sw, lmax, mmax, n):
     return grid, nlon, mt, M, nfreq_used, nharm
 
 
-def kernel_cost(name, K, M, nharm, npoints, nfreq_used, nlon, split):
-    """Algorithmic (bytes, flops, peak flop/s) of one launch of each kernel."""
-    fb = nfreq_used * K * 16  # Fourier rows handed between stages (float4 per column)
-    peak = F16_PEAK / 3 if split else FP32_PEAK  # 3 MFMA products per algorithmic product
+MFMA_PRODUCTS = {"fp32": 1, "fp16x3": 3, "bf16x3": 6}  # MFMA products per algorithmic product
+
+
+def kernel_cost(name, K, M, nharm, n, npoints, nlon, precision):
+    """Algorithmic (bytes, flops, peak flop/s) of one launch of each stage kernel: the
+    round trip's compulsory bytes B_rt = 2 nharm K 8 + 2 npoints K 4 + 2 nharm n 4 split by
+    stage (Legendre synthesis: spectral input + FP32 Λ; FFT synthesis: grid output; FFT
+    analysis: grid input; Legendre analysis: FP32 Λ + spectral output).  The Fourier rows
+    handed between the stages are intermediates and not counted."""
     if name.startswith("legendre"):
-        lam = M * 4  # Λ of the contributing rings, once per launch (FP32 or hi+lo 16-bit)
-        byts = nharm * K * 8 + lam + fb
-        flops = 4.0 * K * M
-        return byts, flops, peak
+        peak = FP32_PEAK if precision == "fp32" else F16_PEAK / MFMA_PRODUCTS[precision]
+        return nharm * K * 8 + nharm * n * 4, 4.0 * K * M, peak
     fft_flops = float(sum(5.0 * N * math.log2(N) for N in nlon)) * K  # N and S rings in one complex FFT
-    return npoints * K * 4 + fb, fft_flops, FP32_PEAK
+    return npoints * K * 4, fft_flops, FP32_PEAK
+
+
+def stage_ms(sw, S, coeffs, field, back, reps):
+    """Mean launch duration (ms) of each stage kernel: HIP events around `reps` back-to-back
+    launches on the launch stream (sht_time_stage), after a round trip that fills the
+    stream's Fourier scratch."""
+    L = sw.lib()
+    st = S._stream()
+    sw.transform_(field, coeffs, S)
+    sw.transform_(back, field, S)
+    out = {}
+    for i, k in enumerate(KERNELS):
+        ms = C.c_double()
+        c = coeffs if i == 0 else back  # the Legendre analysis writes `back`
+        check = L.sht_time_stage(S._plan, i, c.data.data_ptr(), S.spectrum.nharm, field.data.data_ptr(),
+                                 S.grid.npoints, c.nlayers, reps, st, C.byref(ms))
+        assert check == 0, L.sht_last_error()
+        out[k] = ms.value
+    return out
 
 
 def run_mode(sw, torch, precision, spectrum, grid, coeffs_host, steps, warmup):
@@ -61,8 +83,7 @@ def run_mode(sw, torch, precision, spectrum, grid, coeffs_host, steps, warmup):
         sw.transform_(field, coeffs, S)
         sw.transform_(back, field, S)
     torch.cuda.synchronize()
-    L = sw.lib()
-    assert L.sht_profile_begin(S._plan, steps) == 0
+    # the timed region: `steps` round trips, synchronised on both sides
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -70,21 +91,10 @@ def run_mode(sw, torch, precision, spectrum, grid, coeffs_host, steps, warmup):
         sw.transform_(back, field, S)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    avg = (C.c_double * 4)()
-    cnt = (C.c_int32 * 4)()
-    assert L.sht_profile_end(S._plan, avg, cnt) == 0
-    # plain timing without profiling events (the throughput that is reported)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        sw.transform_(field, coeffs, S)
-        sw.transform_(back, field, S)
-    torch.cuda.synchronize()
-    dt_plain = time.perf_counter() - t0
     out = back.data.cpu().numpy().astype(np.complex128)
+    kms = stage_ms(sw, S, coeffs, field, back, max(steps, 20))
     S.close()
-    return {"s_per_step": dt_plain / steps, "s_per_step_profiled": dt / steps,
-            "kernel_ms": {KERNELS[i]: avg[i] for i in range(4)}, "roundtrip": out, "tuning": tuning}
+    return {"s_per_step": dt / steps, "kernel_ms": kms, "roundtrip": out, "tuning": tuning}
 
 
 def time_loop(torch, fn, steps, warmup):
@@ -238,7 +248,7 @@ def bench_c4(sw, torch, steps, warmup):
     return res
 
 
-def bench_c5(sw, seeds=40, heldout=8, steps=3000):
+def bench_c5(sw, seeds=200, heldout=8, steps=3000):
     """Dry primitive-equation step at T31 L8 with the learned column correction: data from
     T127 L8 runs (zonal jets, N(0, 1 K) noise), MLP training on the engine's kernels, and the
     A/B scores of the corrected coarse model against the plain one on held-out runs."""
@@ -276,40 +286,71 @@ def bench_c5(sw, seeds=40, heldout=8, steps=3000):
             "step_ms": ms}
 
 
+def profile_table(name):
+    """Rows `kernel precision value...` of a committed rocprofv3 summary under profiles/."""
+    path = os.path.join(REPO, "profiles", name)
+    rows = {}
+    if os.path.exists(path):
+        for line in open(path):
+            f = line.split()
+            if len(f) >= 3 and not line.startswith("#"):
+                for prec in f[1].split("+"):
+                    rows[(f[0], prec)] = f[2:]
+    return rows
+
+
 def measured_traffic(kernel, precision):
     """HBM bytes per launch (fetch + write) of `kernel` from the committed rocprofv3
     counter summary (FETCH_SIZE / WRITE_SIZE, scaled by a plain copy), or None."""
-    path = os.path.join(REPO, "profiles", "traffic_t255_k64.txt")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        f = line.split()
-        if len(f) == 4 and not line.startswith("#") and f[0] == kernel and precision in f[1].split("+"):
-            return int(f[2]) + int(f[3])
-    return None
+    r = profile_table("traffic_t255_k64.txt").get((kernel, precision))
+    return int(r[0]) + int(r[1]) if r else None
+
+
+def profiled_ms(kernel, precision):
+    """Mean GPU duration (ms) of `kernel` in the committed rocprofv3 kernel-trace summary."""
+    r = profile_table("kernel_times_t255_k64.txt").get((kernel, precision))
+    return float(r[0]) * 1e-3 if r else None
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(spectrum_o, grid_o, coeffs, seconds=10.0):
-    """The oracle's FP32 C restatement of the reference CPU path on the host cores, timed on
-    whole round trips of the same workload until `seconds` have elapsed (at least one)."""
-    from oracle.cref import CRefTransform, lib as oracle_lib
+    """The oracle's FP32 C restatement of the reference CPU path, timed on whole round trips
+    of the same workload until `seconds` have elapsed (at least one): on the host cores this
+    process may use, capped at 16 (one GPU's share of the host), and on one core."""
+    from oracle.cref import CRefTransform
     from oracle.sht_oracle import OracleTransform
     o = OracleTransform(spectrum_o, grid_o)
     c = CRefTransform(o)
-    threads = min(16, os.cpu_count() or 1)
+    avail = len(os.sched_getaffinity(0))
+    threads = min(16, avail)
     a = coeffs.astype(np.complex64)
-    c.analysis(c.synthesis(a, np.float32, nthreads=threads), np.float32, nthreads=threads)  # warm
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        f = c.synthesis(a, np.float32, nthreads=threads)
-        c.analysis(f, np.float32, nthreads=threads)
-        n += 1
-        if time.perf_counter() - t0 >= seconds:
-            break
-    dt = time.perf_counter() - t0
+
+    def rate(nt, secs):
+        c.analysis(c.synthesis(a, np.float32, nthreads=nt), np.float32, nthreads=nt)  # warm
+        n, t0 = 0, time.perf_counter()
+        while True:
+            c.analysis(c.synthesis(a, np.float32, nthreads=nt), np.float32, nthreads=nt)
+            n += 1
+            if time.perf_counter() - t0 >= secs:
+                break
+        return n, time.perf_counter() - t0
+    n, dt = rate(threads, seconds)
+    n1, dt1 = rate(1, seconds)
     return {"value": n / dt, "unit": "round-trips/s", "cores": threads, "kind": "port",
-            "sample": f"{n} full T255 K={coeffs.shape[0]} FP32 round trips ({dt:.1f} s, OpenMP {threads} threads)"}
+            "sample": f"{n} full T255 K={coeffs.shape[0]} FP32 round trips ({dt:.1f} s, OpenMP {threads} threads; "
+                      f"{avail} host cores available, {cpu_model()})",
+            "single_thread_value": n1 / dt1,
+            "single_thread_sample": f"{n1} round trips in {dt1:.1f} s on one thread",
+            "cpu_model": cpu_model(), "host_cores_available": avail}
 
 
 def main():
@@ -321,7 +362,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="headline workload only (for profiling)")
-    ap.add_argument("--c5-seeds", type=int, default=40)
+    ap.add_argument("--c5-seeds", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run three plain 256 MiB device copies (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -345,7 +386,7 @@ def main():
     coeffs = coeffs.astype(np.complex64).astype(np.complex128)  # the FP32 values the GPU sees
 
     res = {p: run_mode(sw, torch, p, spectrum, grid, coeffs, args.steps, args.warmup)
-           for p in dict.fromkeys([args.precision, "fp32"])}
+           for p in dict.fromkeys([args.precision, "fp32", "fp16x3", "bf16x3"])}
     head = res[args.precision]
     # the other GPU workloads run before any host-heavy work (the FP64 oracle, the CPU
     # baseline), so host threads do not slow their launches
@@ -376,27 +417,37 @@ def main():
                 worst = max(worst, e)
         errs[p] = {"worst": worst, "per_field_band": per}
 
-    split = args.precision != "fp32"
-    costs = {k: kernel_cost(k, K, M, nharm, grid.npoints, nfreq_used, nlon, split and k.startswith("legendre"))
-             for k in KERNELS}
+    def roofline(kernel, precision, ms):
+        byts, flops, peak = kernel_cost(kernel, K, M, nharm, n, grid.npoints, nlon, precision)
+        t = ms * 1e-3
+        if byts / HBM_PEAK >= flops / peak:
+            r = {"bound": "hbm", "achieved": byts / t / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s"}
+        else:
+            r = {"bound": "mfma", "achieved": flops / t / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s"}
+        r["frac"] = r["achieved"] / r["peak"]
+        r["bytes"] = byts
+        return r
+
     kms = head["kernel_ms"]
     dominant = max(KERNELS, key=lambda k: kms[k])
-    byts, flops, peak = costs[dominant]
-    t = kms[dominant] * 1e-3
-    if byts / HBM_PEAK >= flops / peak:
-        roof = {"bound": "hbm", "achieved": byts / t / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s"}
-    else:
-        roof = {"bound": "mfma", "achieved": flops / t / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s"}
-    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof = roofline(dominant, args.precision, kms[dominant])
     roof["traffic"] = measured_traffic(dominant, args.precision)
     roof["traffic_source"] = "profiles/traffic_t255_k64.txt (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
     roof["kernel"] = dominant
     roof["kernel_ms"] = kms[dominant]
+    pm = profiled_ms(dominant, args.precision)
+    if pm:  # the same figure from the committed rocprofv3 kernel-trace mean
+        roof["rocprof_kernel_ms"] = pm
+        roof["rocprof_frac"] = roofline(dominant, args.precision, pm)["frac"]
+    stage_roofline = {p: {k: {"ms": r["kernel_ms"][k], "frac": roofline(k, p, r["kernel_ms"][k])["frac"],
+                              "bound": roofline(k, p, r["kernel_ms"][k])["bound"]} for k in KERNELS}
+                      for p, r in res.items()}
 
     # whole round trip against its compulsory bytes / flops
     B_rt = 2 * nharm * K * 8 + 2 * grid.npoints * K * 4 + 2 * nharm * n * 4
     F_rt = 2 * 4.0 * K * M
-    t_roof = max(B_rt / HBM_PEAK, F_rt / (F16_PEAK / 3 if split else FP32_PEAK))
+    P_leg = FP32_PEAK if args.precision == "fp32" else F16_PEAK / MFMA_PRODUCTS[args.precision]
+    t_roof = max(B_rt / HBM_PEAK, F_rt / P_leg)
 
     value = 1.0 / head["s_per_step"]
     out = {
@@ -416,9 +467,11 @@ def main():
                                "16 layers x 4 fields (K=64)", "lmax": lmax, "mmax": mmax, "nlat_half": n, "K": K,
                    "precision": args.precision},
         "roofline": roof,
+        "round_trips_per_s": {p: 1.0 / r["s_per_step"] for p, r in res.items()},
         "fp32_round_trips_per_s": 1.0 / res["fp32"]["s_per_step"],
         "speedup_vs_fp32": res["fp32"]["s_per_step"] / head["s_per_step"],
         "kernel_ms": {p: r["kernel_ms"] for p, r in res.items()},
+        "stage_roofline": stage_roofline,
         "autotuned": {p: r["tuning"] for p, r in res.items()},
         "roundtrip_roofline_frac": t_roof / head["s_per_step"],
         "max_rel_err_vs_fp64": {p: e["worst"] for p, e in errs.items()},

@@ -57,7 +57,7 @@ extern "C" {
 /* arithmetic of the Legendre contraction (FFT and all other steps are FP32) */
 #define SHT_PREC_FP32 0      /* FP32 products, FP32 accumulation (MFMA f32) */
 #define SHT_PREC_FP16X3 1    /* scaled FP16 hi/lo split, 3 products, FP32 accumulation */
-#define SHT_PREC_BF16X3 2    /* BF16 hi/lo split, 3 products, FP32 accumulation */
+#define SHT_PREC_BF16X3 2    /* three-term BF16 split (hi+mid+lo), 6 products, FP32 accumulation */
 
 typedef struct sht_desc {
     int32_t grid;        /* SHT_GRID_* */
@@ -85,7 +85,11 @@ typedef struct sht_sizes {
 /* Precompute a plan (SpectralTransform(spectrum, grid; NF=Float32, nlayers,
  * LegendreShortcut), spectral_transform.jl:118-302): Gaussian latitudes and weights and the
  * Legendre polynomials in FP64, stored in the layout the precision mode needs; FFT
- * factorisations and twiddles; scratch for nlayers columns.  Blocking, host side. */
+ * factorisations and twiddles; scratch for nlayers columns.  Blocking, host side.
+ * Scratch is per (plan, stream): the null stream's is allocated here, another stream's on
+ * its first call (at most 16 streams per plan), so calls on different streams may overlap
+ * on one plan, as the reference's explicit-scratch transform! permits
+ * (spectral_transform.jl:522-531). */
 int sht_plan_create(const sht_desc *desc, sht_plan **plan);
 int sht_plan_destroy(sht_plan *plan);
 int sht_plan_sizes(const sht_desc *desc, sht_sizes *sizes);
@@ -101,8 +105,8 @@ int sht_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *fie
  * (SpeedyTransforms/src/matrix_transform.jl:56-260; the GPU default for truncations up to 64,
  * SpeedyWeather/src/dynamics/spectral_grid.jl:343-349).  Builds a regular plan plus the two
  * dense transform matrices (FP64-computed, stored FP32); sht_synthesis and sht_analysis then
- * run as one FP32 GEMM each (a split-K GEMM of this library; the plan's workspace makes
- * concurrent dense calls on one plan race), every other entry point as on a regular plan.
+ * run as one FP32 GEMM each (a split-K GEMM of this library, its partial sums in the
+ * stream's scratch), every other entry point as on a regular plan.
  * SHT_ERR_UNSUPPORTED if the matrices would exceed 4 GiB. */
 int sht_plan_create_matrix(const sht_desc *desc, sht_plan **plan);
 
@@ -217,18 +221,32 @@ int sht_mlp_train_step(sht_mlp *mlp, const float *x, const float *t, int32_t B, 
                        void *stream);
 int sht_mlp_destroy(sht_mlp *mlp);
 
-/* Autotuner (no reference counterpart; tile and order choices per size class): on K
- * columns of scratch data, times the ring FFT's factorisation policies (first-pass MFMA
- * threshold 11 or 17, powers of two as 8s or 4s, small odd radices first or last) and the
- * launch orders (Legendre tiles grouped per order m on one XCD or plain heaviest-first;
- * FFT synthesis column groups paired or not), keeps the fastest in the plan and caches it
- * per (grid, nlat_half, lmax, mmax, nlayers, precision, K) for later plans in the process.
- * Synchronises the stream; results are unchanged up to FP32 rounding of the FFT order. */
+/* Autotuner (no reference counterpart; kernel, tile and order choices per size class): on K
+ * columns of scratch data, times the Fourier stage's kernels -- the LDS ring FFT under each
+ * factorisation policy (first-pass MFMA threshold 11 or 17, powers of two as 8s or 4s, small
+ * odd radices first or last) and the ring DFT on MFMA (FP32, or FP16 hi/lo on split plans)
+ * with 4 or 8 columns per workgroup -- then the launch orders (Legendre tiles grouped per
+ * order m on one XCD or plain heaviest-first; LDS-FFT synthesis column groups paired or not),
+ * keeps the fastest in the plan and caches it per (grid, nlat_half, lmax, mmax, nlayers,
+ * precision, K) for later plans in the process.  Synchronises the stream; results are
+ * unchanged up to the rounding of the Fourier stage's arithmetic. */
+#define SHT_FOURIER_RADIX 0   /* mixed-radix LDS FFT, FP32 */
+#define SHT_FOURIER_DFT32 1   /* radix-4 pass + p-point DFT on MFMA, FP32 products */
+#define SHT_FOURIER_DFT16 2   /* the same with scaled FP16 hi/lo products (split plans) */
 int sht_plan_autotune(sht_plan *plan, int32_t K, void *stream);
-/* out[6] = {tuned, MFMA threshold, eights, odd_first, Legendre XCD grouping, FFT pairing} */
+/* out[8] = {tuned, MFMA threshold, eights, odd_first, Legendre XCD grouping, FFT pairing,
+ * Fourier kernel (SHT_FOURIER_*), DFT columns per workgroup} */
 int sht_plan_tuning(const sht_plan *plan, int32_t *out);
-/* pins a choice: in[5] = {MFMA threshold (11 or 17), eights, odd_first, Legendre XCD
- * grouping, FFT pairing}; synchronises the device */
+/* Tuning cache file (one flat JSON object per line): save writes every choice the autotuner
+ * made or loaded in this process; load merges a file's choices into the cache, so a later
+ * sht_plan_autotune of the same size class applies them without timing (nloaded may be
+ * null).  SHT_ERR_ARG on unreadable files or malformed lines (the valid lines still load). */
+int sht_tune_cache_save(const char *path);
+int sht_tune_cache_load(const char *path, int32_t *nloaded);
+/* pins a choice: in[7] = {MFMA threshold (11 or 17), eights, odd_first, Legendre XCD
+ * grouping, FFT pairing, Fourier kernel (SHT_FOURIER_*), DFT columns per workgroup (4 or 8)}.  Every choice's tables are built at plan creation, so this
+ * changes host state only: calls already queued keep what they were launched with.  Not
+ * to be called concurrently with other calls on the plan from another host thread. */
 int sht_plan_set_tuning(sht_plan *plan, const int32_t *in);
 
 /* Per-kernel timing (no reference counterpart; the reference profiled ad hoc with
@@ -242,6 +260,14 @@ int sht_plan_set_tuning(sht_plan *plan, const int32_t *in);
 #define SHT_KERNEL_LEGENDRE_ANALYSIS 3
 #define SHT_NKERNELS 4
 int sht_profile_begin(sht_plan *plan, int32_t max_calls);
+/* Mean duration (ms) of one launch of a single stage kernel (SHT_KERNEL_*), timed with HIP
+ * events around `reps` back-to-back launches on `stream` (after one untimed launch), on
+ * K columns with the plan's chosen launch orders.  The Legendre synthesis reads coeffs and
+ * the FFT synthesis writes field, the FFT analysis reads field and the Legendre analysis
+ * writes coeffs; the Fourier stages hand over through the stream's scratch, so run a
+ * transform on the stream first for meaningful values.  Blocks until the launches end. */
+int sht_time_stage(sht_plan *plan, int32_t stage, void *coeffs, int64_t ld_spec, void *field, int64_t ld_grid,
+                   int32_t K, int32_t reps, void *stream, double *avg_ms);
 int sht_profile_end(sht_plan *plan, double *avg_ms, int32_t *counts);
 
 /* Host-only precompute exposed for verification (no GPU needed). */

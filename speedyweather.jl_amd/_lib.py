@@ -28,7 +28,8 @@ EXPORTS = (
     "sht_mlp_get_params", "sht_mlp_forward", "sht_mlp_gradient", "sht_mlp_train_step", "sht_mlp_destroy",
     "sht_plan_autotune", "sht_plan_tuning", "sht_plan_set_tuning", "sht_spec2grid_pullback",
     "sht_grid2spec_pullback", "sht_divergence", "sht_laplace", "sht_gradient", "sht_plan_create_matrix",
-    "sht_synthesis_laplace", "sht_analysis_laplace",
+    "sht_synthesis_laplace", "sht_analysis_laplace", "sht_time_stage", "sht_tune_cache_save",
+    "sht_tune_cache_load",
 )
 
 
@@ -106,6 +107,9 @@ def lib() -> C.CDLL:
         L.sht_plan_set_tuning.argtypes = [vp, vp]
         L.sht_profile_begin.argtypes = [vp, i32]
         L.sht_profile_end.argtypes = [vp, vp, vp]
+        L.sht_time_stage.argtypes = [vp, i32, vp, i64, vp, i64, i32, i32, vp, vp]
+        L.sht_tune_cache_save.argtypes = [C.c_char_p]
+        L.sht_tune_cache_load.argtypes = [C.c_char_p, vp]
         L.sht_last_error.restype = C.c_char_p
         L.sht_version.restype = C.c_char_p
         for name in EXPORTS:

@@ -5,6 +5,8 @@
 
 #include <string>
 
+#include "../../include/sht.h"
+
 namespace sht {
 
 // Column group: 32 complex columns = 64 real MFMA columns per Legendre workgroup.
@@ -41,6 +43,9 @@ constexpr int kMfmaTilesPerWave = 2;
 constexpr int kMaxMfmaRadix = 2 * 16 * kMfmaTilesPerWave * (kFFTThreads / 64) - 1;
 
 enum Mode { kFP32 = 0, kFP16x3 = 1, kBF16x3 = 2 };
+// 16-bit parts per operand value: scaled FP16 hi + lo (three products), BF16 hi + mid + lo
+// (six products: every term down to 2^-16 of the leading one).
+constexpr int split_parts(int mode) { return mode == kBF16x3 ? 3 : 2; }
 
 // One workgroup's work item.  Legendre stages: order m, ring group (synthesis) or degree
 // group (analysis), column group cg.  FFT stages: m = ring, blk = column group.  call
@@ -86,6 +91,50 @@ struct CallArgs {
 
 enum Stage { kLegendreSynthesis = 0, kFFTSynthesis = 1, kFFTAnalysis = 2, kLegendreAnalysis = 3 };
 
+// Fourier-stage kernels: the mixed-radix LDS FFT, or the ring DFT on MFMA (a radix-4 pass
+// on the vector units and a p-point DFT as two GEMMs, FP32 or scaled FP16 hi/lo products),
+// with 4 or 8 columns per workgroup.
+enum FourierKind { kFourierRadix = 0, kFourierDft32 = 1, kFourierDft16 = 2 };
+constexpr int kDftChunk32 = 16;  // contraction values per FP32 fragment load (4 K-steps of 4)
+constexpr int kDftChunk16 = 32;  // contraction values per FP16 MFMA K-step
+constexpr float kDftAScale = 16384.f;  // FP16 tables hold 2^14 cos, 2^14 sin
+// Rows of the DFT GEMM an analysis of ring (p, mt) needs: r <= min(p/2, mt/4 + 1).
+__host__ __device__ inline int dft_rows_analysis(int p, int mt) {
+    const int a = p / 2, b = mt / 4 + 1;
+    return (a < b ? a : b) + 1;
+}
+// Contraction length of a synthesis: t <= min((p-1)/2, (mt+3)/4) (orders above mt are zero).
+__host__ __device__ inline int dft_k_synthesis(int p, int mt) {
+    const int a = (p - 1) / 2, b = (mt + 3) / 4;
+    return a < b ? a : b;
+}
+// Padded contraction length and LDS row strides (elements) of the staged B operands.
+__host__ __device__ inline int dft_kpad(int K, bool f16) {
+    const int c = f16 ? kDftChunk16 : kDftChunk32;
+    return (K + c - 1) / c * c;
+}
+__host__ __device__ inline int dft_bstride(int kpad, bool f16) { return kpad + (f16 ? 8 : 4); }
+// Dynamic LDS bytes of one DFT workgroup (cg columns) for a ring (p, mt); the launch takes
+// the maximum over the rings.  Analysis: S, D operands then the staged outputs F (aliased);
+// synthesis: staged Fourier rows, S, D operands, then the outputs Y (aliased over both).
+__host__ __device__ inline int dft_smem_bytes(bool synthesis, bool f16, int cg, int p, int mt) {
+    const int h = (p - 1) / 2;
+    const int K = synthesis ? dft_k_synthesis(p, mt) : h;
+    const int kp = dft_kpad(K < 1 ? 1 : K, f16);
+    const int bs = dft_bstride(kp, f16);
+    const int cols = 8 * cg;
+    const int bbytes = 2 * cols * bs * (f16 ? 4 : 4);  // S and D: float, or FP16 hi + lo
+    const int aux = cg * 4 * 2 * 8 + cg * 4 * 4;        // X0, Xh per (column, m2), scales
+    if (!synthesis) {
+        const int fbytes = cg * (2 * mt + 2) * 8;
+        return (bbytes > fbytes ? bbytes : fbytes) + aux;
+    }
+    const int zbytes = cg * (mt + 1) * 16;
+    const int ybytes = cg * p * 4 * 8;
+    const int a = zbytes + bbytes;
+    return (a > ybytes ? a : ybytes) + aux;
+}
+
 // One pass of a ring FFT (radix R, l = product of the earlier radices, M = N / (l R)) with
 // the multipliers of FastDiv for M and R precomputed, so the kernels divide by neither.
 struct FFTPass {
@@ -123,15 +172,18 @@ struct DevPlan {
     const long long *syn_soff;   // offset of m's row scales (synthesis, per ring)
     const long long *ana_soff;   // offset of m's row scales (analysis, per degree)
 
-    // Λ in MFMA-fragment order.  FP32 mode: float arrays.  Split modes: hi and lo
-    // arrays of 16-bit values (FP16 or BF16) and inverse row scales (FP16 only).
+    // Λ in MFMA-fragment order.  FP32 mode: float arrays.  Split modes: arrays of 16-bit
+    // parts, v = hi + lo (scaled FP16) or v = hi + mid + lo (BF16, three terms), and
+    // inverse row scales (FP16 only).
     const float *syn_f32;
     const void *syn_hi;
     const void *syn_lo;
+    const void *syn_mid;
     const float *syn_sinv;
     const float *ana_f32;
     const void *ana_hi;
     const void *ana_lo;
+    const void *ana_mid;
     const float *ana_sinv;
 
     // FFT tables
@@ -139,6 +191,15 @@ struct DevPlan {
     const long long *tw_off;
     const FFTPass *passes;  // kMaxFactors passes per ring, in order
     const int *nfac;
+    // Ring DFT on MFMA (rings of N = 4p points): per ring, the cos / sin matrices
+    // cos(2 pi r t / p), sin(2 pi r t / p), rows r = 0..p/2 (tiles of 16), columns
+    // t = 1..(p-1)/2, in MFMA fragment order: FP32 (dft32, chunks of 16 t, a float4 of cos
+    // then one of sin per lane) and, in the split modes, FP16 hi / lo of 2^14 times the value
+    // (dft16, chunks of 32 t, per lane 8 cos hi, 8 cos lo, 8 sin hi, 8 sin lo).
+    const float *dft32;
+    const long long *dft32_off;
+    const unsigned short *dft16;
+    const long long *dft16_off;
 
     // UV_from_vordiv factors (per packed lm): x = -m/(l(l+1)), u1 = eps_lm / l,
     // u2 = eps_{l+1,m} / (l+1) (0 on the last row)
@@ -155,6 +216,14 @@ struct DevPlan {
 // Records the message sht_last_error() returns on this thread.
 void set_last_error(const std::string &msg);
 
+// A plain transform call (SHT_DIR_SYNTHESIS or SHT_DIR_ANALYSIS, the conventions of
+// sht_synthesis / sht_analysis) whose kernels use `scratch` (at least plan_scratch_bytes(plan,
+// K) bytes of device memory) instead of the stream's scratch of the plan.
+int transform_with_scratch(sht_plan *plan, int direction, const void *in, int64_t ld_in, void *out, int64_t ld_out,
+                           int32_t K, int32_t unscale_coslat, void *scratch, void *stream);
+size_t plan_scratch_bytes(const sht_plan *plan, int K);
+int plan_device(const sht_plan *plan);
+
 // Kernel entry points.  Every stage kernel takes
 //   (const DevPlan *plans, CallArgs single, const CallArgs *calls, const Tile *tiles)
 // on a 1-D grid.  batch = false: the single call's arguments by value; the Legendre
@@ -164,6 +233,9 @@ const void *legendre_kernel(int stage, int mode, bool batch);
 // accumulate: the single-call synthesis that adds into its output (pullbacks); batches
 // never accumulate.
 const void *fft_kernel(int stage, bool batch, bool accumulate = false);
+// Ring DFT on MFMA (sht_dft.hip): FP16 hi/lo or FP32 products, cg = 4 or 8 columns per
+// workgroup; single calls launch n * ceil(K / cg) blocks, dynamic LDS dft_smem_bytes.
+const void *dft_kernel(int stage, bool f16, int cg, bool batch, bool accumulate = false);
 // Spectral gradient operators (one kernel, op code; flags: kOp* bits).
 enum SpectralOp { kOpDivergence = 0, kOpCurl = 1, kOpLaplace = 2, kOpGradient = 3 };
 constexpr int kOpFlip = 1, kOpAdd = 2, kOpInverse = 4;

@@ -4,14 +4,15 @@
 // Λ_m (degrees x rings) and the K spectral / Fourier columns, split into even and odd
 // degrees l-m so one contraction serves both hemispheres (the even/odd symmetry of
 // SpeedyTransforms/src/legendre.jl:2-33,104-126).  Runs on MFMA: v_mfma_f32_16x16x4_f32
-// in FP32 mode; v_mfma_f32_16x16x32_{f16,bf16} with a hi/lo split of both operands (three
-// products, FP32 accumulation) in the split modes.
+// in FP32 mode; v_mfma_f32_16x16x32_{f16,bf16} in the split modes, FP32 accumulation:
+// scaled FP16 hi/lo parts of both operands (three products), or BF16 hi/mid/lo parts of
+// both operands (six products, every term down to 2^-16 of the leading one).
 //
 // The contraction dimension is streamed in chunks of 32 (degrees per parity, or rings):
 // the next chunk's operands are loaded into registers while the current chunk's MFMAs
 // run, and the staged operand tile in LDS is double-buffered (one barrier per chunk).
 //
-// Fourier buffer (owned by the plan): row (fboff[j] + m) holds, for every column k, a
+// Fourier buffer (the scratch of the call's stream): row (fboff[j] + m) holds, for every column k, a
 // float4 (N.re, N.im, S.re, S.im).  Synthesis writes the northern/southern Legendre sums
 // g_N, g_S there (legendre.jl:84-91); analysis reads the even/odd combinations
 // f_N + f_S, f_N - f_S (legendre.jl:176-183, the solid angle folded into Λ).
@@ -31,12 +32,23 @@ constexpr int kTileH = 2 * kColsReal * kRowH;
 
 template <int MODE>
 struct LegSmem {
-    static constexpr int bytes = MODE == kFP32 ? 2 * kTileF * 4 : 2 * 2 * kTileH * 2;
+    static constexpr int bytes = MODE == kFP32 ? 2 * kTileF * 4 : 2 * split_parts(MODE) * kTileH * 2;
 };
 
+template <int MODE>
+using V8of = typename Split<MODE == kFP32 ? kFP16x3 : MODE>::V;
+
+// Part q of a split Λ array set: hi (q = 0), then mid (BF16, q = 1), then lo (the last part).
+template <int MODE>
+__device__ __forceinline__ const void *lam_part(const void *hi, const void *mid, const void *lo, int q) {
+    return q == 0 ? hi : (q == split_parts(MODE) - 1 ? lo : mid);
+}
+
 // Stages 4 contraction values x (re, im) of one complex column (real columns 2kk, 2kk+1)
-// and parity p into the chunk tile: FP32 as is; split modes as hi/lo with one power-of-two
-// scale per (parity, real column) over the 32 values the 8 lanes of a column group hold.
+// and parity p into the chunk tile: FP32 as is; split modes as 16-bit parts (tile q of the
+// buffer holds part q): scaled FP16 hi/lo with one power-of-two scale per (parity, real
+// column) over the 32 values the 8 lanes of a column group hold, or BF16 hi/mid/lo, whose
+// sum is the FP32 value exactly.
 template <int MODE>
 __device__ inline void stage4(unsigned char *sm, int buf, int p, int kk, int g, const float (&re)[4],
                               const float (&im)[4]) {
@@ -46,8 +58,8 @@ __device__ inline void stage4(unsigned char *sm, int buf, int p, int kk, int g, 
         *reinterpret_cast<float4 *>(B + (2 * kk + 1) * kRowF + 4 * g) = make_float4(im[0], im[1], im[2], im[3]);
     } else {
         typedef typename Split<MODE>::T T16;
-        u16 *Bh = reinterpret_cast<u16 *>(sm) + (buf * 2 + 0) * kTileH + p * kColsReal * kRowH;
-        u16 *Bl = reinterpret_cast<u16 *>(sm) + (buf * 2 + 1) * kTileH + p * kColsReal * kRowH;
+        constexpr int NP = split_parts(MODE);
+        u16 *B0 = reinterpret_cast<u16 *>(sm) + (buf * NP) * kTileH + p * kColsReal * kRowH;
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             const float *x = c == 0 ? re : im;
@@ -60,19 +72,21 @@ __device__ inline void stage4(unsigned char *sm, int buf, int p, int kk, int g, 
                 amax = fmaxf(amax, lane_mirror8(amax));
                 fp16_scale(amax, s, sinv);
             }
-            u16x4 hi, lo;
+            u16x4 part[NP];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const float v = x[e] * s;
                 const T16 h = (T16)v;
-                const T16 l = (T16)(v - (float)h);
-                hi[e] = __builtin_bit_cast(u16, h);
-                lo[e] = __builtin_bit_cast(u16, l);
+                const float r = v - (float)h;  // exact
+                const T16 m = (T16)r;
+                part[0][e] = __builtin_bit_cast(u16, h);
+                part[1][e] = __builtin_bit_cast(u16, m);
+                if constexpr (NP == 3) part[2][e] = __builtin_bit_cast(u16, (T16)(r - (float)m));
             }
             const int col = 2 * kk + c;
-            *reinterpret_cast<u16x4 *>(Bh + col * kRowH + 4 * g) = hi;
-            *reinterpret_cast<u16x4 *>(Bl + col * kRowH + 4 * g) = lo;
-            if (g == 0) *reinterpret_cast<float *>(Bh + col * kRowH + kChunk) = sinv;
+#pragma unroll
+            for (int q = 0; q < NP; ++q) *reinterpret_cast<u16x4 *>(B0 + q * kTileH + col * kRowH + 4 * g) = part[q];
+            if (MODE == kFP16x3 && g == 0) *reinterpret_cast<float *>(B0 + col * kRowH + kChunk) = sinv;
         }
     }
 }
@@ -80,11 +94,12 @@ __device__ inline void stage4(unsigned char *sm, int buf, int p, int kk, int g, 
 // acc[t] += A_t (16 rows x 32 contraction, this lane's fragment of row tile t < TPW) x B
 // tile columns cb*16..cb*16+15 of parity p; each B fragment is read once for all TPW row
 // tiles.  FP32: af[t][8] holds the lane's A values of tile t for the 8 K-steps of 4.
+// Split modes: ap[t][q] holds part q of the lane's A fragment; FP16 sums lo·hi, hi·lo,
+// hi·hi per chunk and adds the chunk rescaled, BF16 accumulates the six products of parts
+// (i, j) with i + j <= 2, smallest first.
 template <int MODE, int NCB, int TPW>
 __device__ inline void mma_chunk(const unsigned char *sm, int buf, int p, int row, int grp, const float (&af)[TPW][8],
-                                 const typename Split<MODE == kFP32 ? kFP16x3 : MODE>::V (&ah)[TPW],
-                                 const typename Split<MODE == kFP32 ? kFP16x3 : MODE>::V (&al)[TPW],
-                                 f4 (&acc)[TPW][NCB]) {
+                                 const V8of<MODE> (&ap)[TPW][3], f4 (&acc)[TPW][NCB]) {
     if constexpr (MODE == kFP32) {
         const float *B = reinterpret_cast<const float *>(sm) + buf * kTileF + p * kColsReal * kRowF;
 #pragma unroll
@@ -97,50 +112,55 @@ __device__ inline void mma_chunk(const unsigned char *sm, int buf, int p, int ro
             }
         }
     } else {
-        typedef typename Split<MODE>::V V8;
-        const u16 *Bh = reinterpret_cast<const u16 *>(sm) + (buf * 2 + 0) * kTileH + p * kColsReal * kRowH;
-        const u16 *Bl = reinterpret_cast<const u16 *>(sm) + (buf * 2 + 1) * kTileH + p * kColsReal * kRowH;
+        typedef V8of<MODE> V8;
+        constexpr int NP = split_parts(MODE);
+        const u16 *B0 = reinterpret_cast<const u16 *>(sm) + (buf * NP) * kTileH + p * kColsReal * kRowH;
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb) {
             const int col = cb * 16 + row;
-            const V8 bh = __builtin_bit_cast(V8, *reinterpret_cast<const u16x8 *>(Bh + col * kRowH + 8 * grp));
-            const V8 bl = __builtin_bit_cast(V8, *reinterpret_cast<const u16x8 *>(Bl + col * kRowH + 8 * grp));
+            V8 b[NP];
+#pragma unroll
+            for (int q = 0; q < NP; ++q)
+                b[q] = __builtin_bit_cast(V8, *reinterpret_cast<const u16x8 *>(B0 + q * kTileH + col * kRowH + 8 * grp));
             if constexpr (MODE == kFP16x3) {
-                const float sc = *reinterpret_cast<const float *>(Bh + col * kRowH + kChunk);
+                const float sc = *reinterpret_cast<const float *>(B0 + col * kRowH + kChunk);
 #pragma unroll
                 for (int t = 0; t < TPW; ++t) {
-                    f4 u = Split<MODE>::mma(al[t], bh, f4{0.f, 0.f, 0.f, 0.f});
-                    u = Split<MODE>::mma(ah[t], bl, u);
-                    u = Split<MODE>::mma(ah[t], bh, u);
+                    f4 u = Split<MODE>::mma(ap[t][1], b[0], f4{0.f, 0.f, 0.f, 0.f});
+                    u = Split<MODE>::mma(ap[t][0], b[1], u);
+                    u = Split<MODE>::mma(ap[t][0], b[0], u);
                     acc[t][cb] += u * sc;
                 }
             } else {
 #pragma unroll
                 for (int t = 0; t < TPW; ++t) {
-                    acc[t][cb] = Split<MODE>::mma(al[t], bh, acc[t][cb]);
-                    acc[t][cb] = Split<MODE>::mma(ah[t], bl, acc[t][cb]);
-                    acc[t][cb] = Split<MODE>::mma(ah[t], bh, acc[t][cb]);
+                    f4 c = acc[t][cb];
+                    c = Split<MODE>::mma(ap[t][2], b[0], c);
+                    c = Split<MODE>::mma(ap[t][0], b[2], c);
+                    c = Split<MODE>::mma(ap[t][1], b[1], c);
+                    c = Split<MODE>::mma(ap[t][1], b[0], c);
+                    c = Split<MODE>::mma(ap[t][0], b[1], c);
+                    acc[t][cb] = Split<MODE>::mma(ap[t][0], b[0], c);
                 }
             }
         }
     }
 }
 
-template <int MODE>
-using V8of = typename Split<MODE == kFP32 ? kFP16x3 : MODE>::V;
-
 // The same for one row tile.
 template <int MODE, int NCB>
 __device__ inline void mma_chunk1(const unsigned char *sm, int buf, int p, int row, int grp, const float (&af)[8],
-                                  V8of<MODE> ah, V8of<MODE> al, f4 (&acc)[NCB]) {
+                                  const V8of<MODE> (&ap)[3], f4 (&acc)[NCB]) {
     float a1[1][8];
 #pragma unroll
     for (int ks = 0; ks < 8; ++ks) a1[0][ks] = af[ks];
-    const V8of<MODE> h1[1] = {ah}, l1[1] = {al};
+    V8of<MODE> p1[1][3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) p1[0][q] = ap[q];
     f4 c1[1][NCB];
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) c1[0][cb] = acc[cb];
-    mma_chunk<MODE, NCB, 1>(sm, buf, p, row, grp, a1, h1, l1, c1);
+    mma_chunk<MODE, NCB, 1>(sm, buf, p, row, grp, a1, p1, c1);
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) acc[cb] = c1[0][cb];
 }
@@ -173,7 +193,7 @@ __device__ inline unsigned char *ep_slot(unsigned char *sm, int rb) {
 // Four workgroups per CU: at most 128 registers per lane, and the per-ring epilogue factors
 // sit in the staging rows' padding, so LDS stays at 40 KiB (36 KiB in FP32).
 template <int MODE, bool BATCH>
-__global__ __launch_bounds__(256, 4) void legendre_synthesis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
+__global__ __launch_bounds__(256, MODE == kBF16x3 ? 2 : 4) void legendre_synthesis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
                                                                  const CallArgs *__restrict__ calls,
                                                                  const Tile *__restrict__ tiles) {
     // the wave index is uniform per wave: keep it (and all tile ownership) scalar
@@ -221,20 +241,23 @@ __global__ __launch_bounds__(256, 4) void legendre_synthesis_kernel(const DevPla
     struct Ops {
         float2 v[8];
         float af[2][8];
-        V8of<MODE> ah[2], al[2];
+        V8of<MODE> ap[2][3];  // [parity][part]
     };
     const int ltile = active ? tile : Jt - 1;
     const long long so = as_global(P.syn_off)[m];
     const SHT_GLOBAL float *Af[2];
-    const SHT_GLOBAL V8of<MODE> *Ah[2], *Al[2];
+    const SHT_GLOBAL V8of<MODE> *Ap[2][3];
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
         if constexpr (MODE == kFP32) {
             Af[p] = as_global(P.syn_f32) + so + (long long)(p * Jt + ltile) * (nch * kChunk) * 16 + grp * 16 + row;
         } else {
             const long long fi = (long long)(p * Jt + ltile) * nch * 64 + lane;
-            Ah[p] = as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.syn_hi) + so)) + fi;
-            Al[p] = as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.syn_lo) + so)) + fi;
+#pragma unroll
+            for (int q = 0; q < split_parts(MODE); ++q)
+                Ap[p][q] = as_global(reinterpret_cast<const V8of<MODE> *>(
+                               static_cast<const u16 *>(lam_part<MODE>(P.syn_hi, P.syn_mid, P.syn_lo, q)) + so)) +
+                           fi;
         }
     }
     auto load = [&](Ops &S, int c) {
@@ -247,8 +270,8 @@ __global__ __launch_bounds__(256, 4) void legendre_synthesis_kernel(const DevPla
 #pragma unroll
                 for (int ks = 0; ks < kChunk / 4; ++ks) S.af[p][ks] = Af[p][c * kChunk * 16 + ks * 64];
             } else {
-                S.ah[p] = Ah[p][c * 64];
-                S.al[p] = Al[p][c * 64];
+#pragma unroll
+                for (int q = 0; q < split_parts(MODE); ++q) S.ap[p][q] = Ap[p][q][c * 64];
             }
         }
     };
@@ -279,7 +302,7 @@ __global__ __launch_bounds__(256, 4) void legendre_synthesis_kernel(const DevPla
         if (active) {
 #pragma unroll
             for (int p = 0; p < 2; ++p)
-                mma_chunk1<MODE, 4>(sm, buf, p, row, grp, cur.af[p], cur.ah[p], cur.al[p], acc[p]);
+                mma_chunk1<MODE, 4>(sm, buf, p, row, grp, cur.af[p], cur.ap[p], acc[p]);
         }
     };
 
@@ -352,7 +375,7 @@ __global__ __launch_bounds__(256, 4) void legendre_synthesis_kernel(const DevPla
 
 // At least three workgroups per CU (at most 168 registers per lane).
 template <int MODE, bool BATCH>
-__global__ __launch_bounds__(256, 3) void legendre_analysis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
+__global__ __launch_bounds__(256, MODE == kBF16x3 ? 2 : 3) void legendre_analysis_kernel(const DevPlan *__restrict__ plans, CallArgs single,
                                                                 const CallArgs *__restrict__ calls,
                                                                 const Tile *__restrict__ tiles) {
     constexpr int TPW = ana_tiles_per_wave(MODE);
@@ -413,19 +436,22 @@ __global__ __launch_bounds__(256, 3) void legendre_analysis_kernel(const DevPlan
     struct Ops {
         float4 v[4];
         float af[TPW][8];
-        V8of<MODE> ah[TPW], al[TPW];
+        V8of<MODE> ap[TPW][3];  // [tile][part]
         int fo[4];  // first Fourier rows of the following chunk's rings (< 2^31 rows)
     };
     const long long ao = as_global(P.ana_off)[m];
     const SHT_GLOBAL float *Af[TPW];
-    const SHT_GLOBAL V8of<MODE> *Ah[TPW], *Al[TPW];
+    const SHT_GLOBAL V8of<MODE> *Ap[TPW][3];
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
         const int lt = lt0 + t;
         Af[t] = as_global(P.ana_f32) + ao + (long long)(p * nt + lt) * (JC * kChunk) * 16 + grp * 16 + row;
         const long long fi = (long long)(p * nt + lt) * JC * 64 + lane;
-        Ah[t] = as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_hi) + ao)) + fi;
-        Al[t] = as_global(reinterpret_cast<const V8of<MODE> *>(static_cast<const u16 *>(P.ana_lo) + ao)) + fi;
+#pragma unroll
+        for (int q = 0; q < split_parts(MODE); ++q)
+            Ap[t][q] = as_global(reinterpret_cast<const V8of<MODE> *>(
+                           static_cast<const u16 *>(lam_part<MODE>(P.ana_hi, P.ana_mid, P.ana_lo, q)) + ao)) +
+                       fi;
     }
     auto load_offsets = [&](int (&fo)[4], int c) {
         c = min(c, JC - 1);
@@ -444,8 +470,8 @@ __global__ __launch_bounds__(256, 3) void legendre_analysis_kernel(const DevPlan
 #pragma unroll
                 for (int ks = 0; ks < kChunk / 4; ++ks) S.af[t][ks] = Af[t][c * kChunk * 16 + ks * 64];
             } else {
-                S.ah[t] = Ah[t][c * 64];
-                S.al[t] = Al[t][c * 64];
+#pragma unroll
+                for (int q = 0; q < split_parts(MODE); ++q) S.ap[t][q] = Ap[t][q][c * 64];
             }
         }
     };
@@ -469,7 +495,7 @@ __global__ __launch_bounds__(256, 3) void legendre_analysis_kernel(const DevPlan
         stage4<MODE>(sm, buf, 1, kk, g, re, im);
         load(nxt, c + 1, cur.fo);
         __syncthreads();
-        mma_chunk<MODE, 4, TPW>(sm, buf, p, row, grp, cur.af, cur.ah, cur.al, acc);
+        mma_chunk<MODE, 4, TPW>(sm, buf, p, row, grp, cur.af, cur.ap, acc);
     };
 
     if (JC > 0) {  // no rings for this order: the coefficients are zero

@@ -228,6 +228,21 @@ struct FFTPolicy {
     bool eights = true;      // powers of two as 8s (then one 4 or 2) rather than 4s
     bool odd_first = false;  // small odd radices before the powers of two
 };
+// Every policy's FFT tables live on the device from plan creation on, so choosing one
+// (autotuning, pinning) changes host state only and never rewrites memory that kernels in
+// flight on other streams may read.
+constexpr int kPolicies = 8;
+inline int policy_index(const FFTPolicy &p) { return (p.mfma_min == 11 ? 4 : 0) + (p.eights ? 2 : 0) + (p.odd_first ? 1 : 0); }
+inline FFTPolicy policy_of(int i) {
+    FFTPolicy p;
+    p.mfma_min = (i & 4) ? 11 : 17;
+    p.eights = (i & 2) != 0;
+    p.odd_first = (i & 1) != 0;
+    return p;
+}
+// Streams one plan serves with their own Fourier scratch (each at most one buffer of the
+// plan's capacity; more streams is an error, not an eviction of scratch in use).
+constexpr int kMaxPlanStreams = 16;
 
 // A launch order for workgroups grouped in units that share data (tiles of one unit run
 // back to back on one XCD): units, heaviest first, go greedily to the least-loaded XCD;
@@ -280,23 +295,35 @@ static std::vector<LaunchUnit> legendre_units(const std::vector<Tile> &t, const 
 struct sht_plan {
     sht_desc d{};
     Geometry g;
-    DevPlan P{};
-    const DevPlan *dP = nullptr;  // device copy of P
+    DevPlan P{};                  // host copy of the device view under the chosen FFT policy
+    const DevPlan *dP = nullptr;  // its device copy (an element of dPv)
+    DevPlan Pv[kPolicies]{};      // device views per FFT policy (same tables but the FFT's)
+    const DevPlan *dPv = nullptr;
+    bool pol_ok[kPolicies] = {};  // the policy factorises every ring
+    int pol = 0;                  // chosen policy
     std::vector<void *> allocs;
     // single-call Legendre launch orders, one per column-group count cgs = 1..max, grouped
     // by order m per XCD ([1]) or plain heavy-first ([0])
     std::vector<const Tile *> syn_tiles[2], ana_tiles[2];
     std::vector<int> nsyn[2], nana[2];
     // tuned choices (sht_plan_autotune): FFT factorisation, Legendre order grouping, FFT
-    // synthesis column-group pairing
-    FFTPolicy fft_policy;
+    // synthesis column-group pairing, Fourier kernel kind and its columns per workgroup
     int leg_xcd = 1, fft_pair = 1;
+    int fourier = sht::kFourierRadix, dft_cg = 4;
     int tuned = 0;
+    // dynamic LDS bytes of the DFT kernels: [synthesis][FP16][8 columns]
+    int dft_smem[2][2][2] = {};
     // host copies for batches: Legendre tiles (heavy first) with their costs, FFT ring order
+    // and cost per policy
     std::vector<Tile> h_syn, h_ana;
-    std::vector<double> h_syn_cost, h_ana_cost, h_ring_cost;
-    std::vector<int> h_ring_order;
-    void *fbuf = nullptr;
+    std::vector<double> h_syn_cost, h_ana_cost, h_ring_cost_v[kPolicies];
+    std::vector<int> h_ring_order_v[kPolicies];
+    // Scratch per stream: the Fourier buffer (fb_bytes), or on matrix plans also the dense
+    // GEMMs' split-K sums (dense_bytes; a call uses one or the other); created on a stream's
+    // first call, freed with the plan.
+    size_t fb_bytes = 0, dense_bytes = 0;
+    std::mutex scratch_mutex;
+    std::vector<std::pair<void *, void *>> scratch;  // (stream, buffer)
     long long device_bytes = 0;
     Profile prof;
     // dense matrix transform (sht_plan_create_matrix): sht_synthesis / sht_analysis run as
@@ -305,7 +332,6 @@ struct sht_plan {
     float *mat_syn = nullptr;      // npoints x 2 nharm: columns (Re B, -Im B) of field = Re(B a)
     float *mat_ana = nullptr;      // 2 nharm x npoints: rows (Re F, Im F) of a = F f
     float *pt_coslat_inv = nullptr;  // 1/cos(lat) per grid point (unscale_coslat)
-    float *dense_work = nullptr;     // split-K partial sums of the dense GEMMs
 };
 
 namespace {
@@ -351,7 +377,30 @@ void free_plan(sht_plan *p) {
     if (!p) return;
     for (hipEvent_t e : p->prof.ev) (void)hipEventDestroy(e);
     for (void *a : p->allocs) (void)hipFree(a);
+    for (auto &sb : p->scratch) (void)hipFree(sb.second);
     delete p;
+}
+
+// The scratch of `stream` on this plan (allocated on the stream's first call), or null
+// with the error recorded.  Calls on one stream are ordered, so they share it safely;
+// calls on different streams never share one.
+void *stream_scratch(sht_plan *p, void *stream) {
+    std::lock_guard<std::mutex> lock(p->scratch_mutex);
+    for (auto &sb : p->scratch)
+        if (sb.first == stream) return sb.second;
+    if ((int)p->scratch.size() >= kMaxPlanStreams) {
+        fail(SHT_ERR_ALLOC, "a plan serves at most " + std::to_string(kMaxPlanStreams) +
+                                " streams with their own scratch");
+        return nullptr;
+    }
+    void *buf = nullptr;
+    const hipError_t e = hipMalloc(&buf, std::max<size_t>(std::max(p->fb_bytes, p->dense_bytes), 256));
+    if (e != hipSuccess) {
+        fail(SHT_ERR_ALLOC, std::string("hipMalloc stream scratch: ") + hipGetErrorString(e));
+        return nullptr;
+    }
+    p->scratch.push_back({stream, buf});
+    return buf;
 }
 
 // Radices of the ring FFT, in pass order: the largest prime >= policy.mfma_min first (the
@@ -443,8 +492,59 @@ int fft_tables(const Geometry &g, const FFTPolicy &pol, std::vector<int> &fac, s
     return SHT_OK;
 }
 
+// Matrices of the ring DFT on MFMA (see DevPlan::dft32 / dft16): for every ring of
+// N = 4p points, cos and sin of 2 pi r t / p for rows r = 0..p/2 and t = 1..(p-1)/2 in FP64,
+// zero outside, stored FP32 and (split plans) as FP16 hi / lo of 2^14 times the value.
+void dft_tables(const Geometry &g, bool split, std::vector<float> &t32, std::vector<long long> &o32,
+                std::vector<uint16_t> &t16, std::vector<long long> &o16) {
+    o32.assign(g.n, 0);
+    o16.assign(g.n, 0);
+    for (int j = 0; j < g.n; ++j) {
+        const int p = g.nlon[j] / 4, h = (p - 1) / 2, R = p / 2 + 1, Rt = (R + 15) / 16;
+        auto cs = [&](int r, int t, double &c, double &sn) {
+            c = sn = 0.0;
+            if (r >= R || t < 1 || t > h) return;
+            const double a = 2.0 * M_PI * (double)(((long long)r * t) % p) / (double)p;
+            c = std::cos(a);
+            sn = std::sin(a);
+        };
+        const int KC = (h + sht::kDftChunk32 - 1) / sht::kDftChunk32;
+        o32[j] = (long long)t32.size();
+        t32.resize(t32.size() + (size_t)Rt * KC * 64 * 8, 0.f);
+        for (int rt = 0; rt < Rt; ++rt)
+            for (int ch = 0; ch < KC; ++ch)
+                for (int lane = 0; lane < 64; ++lane)
+                    for (int st = 0; st < 4; ++st) {
+                        double c, sn;
+                        cs(16 * rt + lane % 16, sht::kDftChunk32 * ch + 4 * (lane / 16) + st + 1, c, sn);
+                        const size_t idx = (size_t)o32[j] + (((size_t)rt * KC + ch) * 64 + lane) * 8;
+                        t32[idx + st] = (float)c;
+                        t32[idx + 4 + st] = (float)sn;
+                    }
+        if (!split) continue;
+        const int KH = (h + sht::kDftChunk16 - 1) / sht::kDftChunk16;
+        o16[j] = (long long)t16.size();
+        t16.resize(t16.size() + (size_t)Rt * KH * 64 * 32, 0);
+        for (int rt = 0; rt < Rt; ++rt)
+            for (int ch = 0; ch < KH; ++ch)
+                for (int lane = 0; lane < 64; ++lane)
+                    for (int e = 0; e < 8; ++e) {
+                        double c, sn;
+                        cs(16 * rt + lane % 16, sht::kDftChunk16 * ch + 8 * (lane / 16) + e + 1, c, sn);
+                        c *= sht::kDftAScale;
+                        sn *= sht::kDftAScale;
+                        const size_t idx = (size_t)o16[j] + (((size_t)rt * KH + ch) * 64 + lane) * 32;
+                        t16[idx + e] = to_half(c);
+                        t16[idx + 8 + e] = to_half(c - half_value(t16[idx + e]));
+                        t16[idx + 16 + e] = to_half(sn);
+                        t16[idx + 24 + e] = to_half(sn - half_value(t16[idx + 16 + e]));
+                    }
+    }
+}
+
 int build_plan(sht_plan *pl) {
     const sht_desc &d = pl->d;
+    long long bytes_dft = 0;
     Geometry &g = pl->g;
     const int n = g.n, lmax = g.lmax, mmax = g.mmax;
     const int mode = d.precision;
@@ -483,23 +583,32 @@ int build_plan(sht_plan *pl) {
         ana_stotal += (long long)2 * nt * 16;
     }
     std::vector<float> syn_f32, ana_f32, syn_sinv, ana_sinv;
-    std::vector<uint16_t> syn_hi, syn_lo, ana_hi, ana_lo;
+    std::vector<uint16_t> syn_hi, syn_lo, syn_mid, ana_hi, ana_lo, ana_mid;
+    const bool three = mode == SHT_PREC_BF16X3;  // BF16: hi + mid + lo
     if (split) {
         syn_hi.assign(syn_total, 0); syn_lo.assign(syn_total, 0);
         ana_hi.assign(ana_total, 0); ana_lo.assign(ana_total, 0);
+        if (three) {
+            syn_mid.assign(syn_total, 0);
+            ana_mid.assign(ana_total, 0);
+        }
         syn_sinv.assign(syn_stotal, 1.f); ana_sinv.assign(ana_stotal, 1.f);
     } else {
         syn_f32.assign(syn_total, 0.f);
         ana_f32.assign(ana_total, 0.f);
     }
-    auto split_store = [&](double v, double s, uint16_t &hi, uint16_t &lo) {
+    // FP16: v s = hi + lo; BF16: v = hi + mid + lo (each part rounds the FP64 remainder)
+    auto split_store = [&](double v, double s, std::vector<uint16_t> &hi, std::vector<uint16_t> &mid,
+                           std::vector<uint16_t> &lo, long long idx) {
         if (mode == SHT_PREC_FP16X3) {
             const double xs = v * s;
-            hi = to_half(xs);
-            lo = to_half(xs - half_value(hi));
+            hi[idx] = to_half(xs);
+            lo[idx] = to_half(xs - half_value(hi[idx]));
         } else {
-            hi = to_bf16(v);
-            lo = to_bf16(v - bf16_value(hi));
+            hi[idx] = to_bf16(v);
+            const double r = v - bf16_value(hi[idx]);
+            mid[idx] = to_bf16(r);
+            lo[idx] = to_bf16(r - bf16_value(mid[idx]));
         }
     };
 
@@ -544,7 +653,7 @@ int build_plan(sht_plan *pl) {
                             const int c = lp / sht::kChunk, kq = lp % sht::kChunk;
                             const int grp = kq / 8, e = kq % 8, lane = grp * 16 + r;
                             const long long idx = syn_off[m] + (((long long)(p * Jt + t) * nch + c) * 64 + lane) * 8 + e;
-                            split_store(v, srow_scale[t * 16 + r], syn_hi[idx], syn_lo[idx]);
+                            split_store(v, srow_scale[t * 16 + r], syn_hi, syn_mid, syn_lo, idx);
                         }
                     }
                 }
@@ -571,16 +680,22 @@ int build_plan(sht_plan *pl) {
                             const int c = jp / sht::kChunk, kq = jp % sht::kChunk;
                             const int grp = kq / 8, e = kq % 8, lane = grp * 16 + r;
                             const long long idx = ana_off[m] + (((long long)(p * nt + lt) * JC + c) * 64 + lane) * 8 + e;
-                            split_store(v, s, ana_hi[idx], ana_lo[idx]);
+                            split_store(v, s, ana_hi, ana_mid, ana_lo, idx);
                         }
                     }
                 }
     }
 
     // ------------------------------------------------------------- FFT tables
-    std::vector<int> fac, nfac, ring_order;
-    std::vector<double> ring_cost;
-    if ((rc = fft_tables(g, pl->fft_policy, fac, nfac, ring_cost, ring_order))) return rc;
+    // every factorisation policy's tables (the default one must exist)
+    std::vector<int> fac_v[kPolicies], nfac_v[kPolicies];
+    const int pol_default = policy_index(FFTPolicy());
+    for (int q = 0; q < kPolicies; ++q) {
+        rc = fft_tables(g, policy_of(q), fac_v[q], nfac_v[q], pl->h_ring_cost_v[q], pl->h_ring_order_v[q]);
+        pl->pol_ok[q] = rc == SHT_OK;
+        if (rc && q == pol_default) return rc;
+    }
+    pl->pol = pol_default;
     std::vector<long long> tw_off(n);
     long long tw_total = 0;
     for (int j = 0; j < n; ++j) {
@@ -634,8 +749,6 @@ int build_plan(sht_plan *pl) {
     pl->h_ana = ana_t;
     pl->h_syn_cost = syn_c;
     pl->h_ana_cost = ana_c;
-    pl->h_ring_order = ring_order;
-    pl->h_ring_cost = ring_cost;
 
     // ---------------------------------------------------- UV_from_vordiv factors
     std::vector<float> uvx(g.nharm), uv1(g.nharm), uv2(g.nharm);
@@ -668,7 +781,6 @@ int build_plan(sht_plan *pl) {
     if ((rc = dev_upload(A, coslat_f, &P.coslat_inv))) return rc;
     std::vector<float> domega_f(dOmega.begin(), dOmega.end());
     if ((rc = dev_upload(A, domega_f, &P.domega))) return rc;
-    if ((rc = dev_upload(A, ring_order, &P.ring_order))) return rc;
     if ((rc = dev_upload(A, g.j0, &P.j0))) return rc;
     if ((rc = dev_upload(A, g.lm_off, &P.lm_off))) return rc;
     if ((rc = dev_upload(A, syn_off, &P.syn_off))) return rc;
@@ -685,6 +797,12 @@ int build_plan(sht_plan *pl) {
         P.ana_hi = p16;
         if ((rc = dev_upload(A, ana_lo, &p16))) return rc;
         P.ana_lo = p16;
+        if (three) {
+            if ((rc = dev_upload(A, syn_mid, &p16))) return rc;
+            P.syn_mid = p16;
+            if ((rc = dev_upload(A, ana_mid, &p16))) return rc;
+            P.ana_mid = p16;
+        }
         if ((rc = dev_upload(A, syn_sinv, &P.syn_sinv))) return rc;
         if ((rc = dev_upload(A, ana_sinv, &P.ana_sinv))) return rc;
     } else {
@@ -693,8 +811,34 @@ int build_plan(sht_plan *pl) {
     }
     if ((rc = dev_upload(A, tw, &P.twiddle))) return rc;
     if ((rc = dev_upload(A, tw_off, &P.tw_off))) return rc;
-    if ((rc = dev_upload(A, pass_table(g, fac, nfac), &P.passes))) return rc;
-    if ((rc = dev_upload(A, nfac, &P.nfac))) return rc;
+    {
+        std::vector<float> t32;
+        std::vector<uint16_t> t16;
+        std::vector<long long> o32, o16;
+        dft_tables(g, split, t32, o32, t16, o16);
+        if ((rc = dev_upload(A, t32, &P.dft32))) return rc;
+        if ((rc = dev_upload(A, o32, &P.dft32_off))) return rc;
+        if (split) {
+            if ((rc = dev_upload(A, t16, &P.dft16))) return rc;
+            if ((rc = dev_upload(A, o16, &P.dft16_off))) return rc;
+        }
+        for (int syn = 0; syn < 2; ++syn)
+            for (int f16 = 0; f16 < 2; ++f16)
+                for (int c8 = 0; c8 < 2; ++c8) {
+                    int mx = 0;
+                    for (int j = 0; j < n; ++j)
+                        mx = std::max(mx, sht::dft_smem_bytes(syn != 0, f16 != 0, c8 ? 8 : 4, g.nlon[j] / 4, g.mt[j]));
+                    pl->dft_smem[syn][f16][c8] = mx;
+                    if (mx > 65536)
+                        for (bool b : {false, true})
+                            for (bool acc : {false, true})
+                                (void)hipFuncSetAttribute(
+                                    sht::dft_kernel(syn ? sht::kFFTSynthesis : sht::kFFTAnalysis, f16 != 0, c8 ? 8 : 4, b,
+                                                    acc),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+                }
+        bytes_dft = (long long)(t32.size() * 4 + t16.size() * 2);
+    }
     if ((rc = dev_upload(A, uvx, &P.uv_x))) return rc;
     if ((rc = dev_upload(A, uv1, &P.uv_1))) return rc;
     if ((rc = dev_upload(A, uv2, &P.uv_2))) return rc;
@@ -732,8 +876,17 @@ int build_plan(sht_plan *pl) {
             }
     }
     {
-        std::vector<DevPlan> one(1, P);
-        if ((rc = dev_upload(A, one, &pl->dP))) return rc;
+        std::vector<DevPlan> views(kPolicies, P);
+        for (int q = 0; q < kPolicies; ++q) {
+            if (!pl->pol_ok[q]) continue;
+            if ((rc = dev_upload(A, pl->h_ring_order_v[q], &views[q].ring_order))) return rc;
+            if ((rc = dev_upload(A, pass_table(g, fac_v[q], nfac_v[q]), &views[q].passes))) return rc;
+            if ((rc = dev_upload(A, nfac_v[q], &views[q].nfac))) return rc;
+            pl->Pv[q] = views[q];
+        }
+        if ((rc = dev_upload(A, views, &pl->dPv))) return rc;
+        pl->P = pl->Pv[pl->pol];
+        pl->dP = pl->dPv + pl->pol;
     }
     if (P.fft_smem > 65536) {
         for (bool b : {false, true})
@@ -744,12 +897,11 @@ int build_plan(sht_plan *pl) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, P.fft_smem);
     }
     const size_t fb_bytes = (size_t)g.fb_rows * d.nlayers * 16;
-    hipError_t e = hipMalloc(&pl->fbuf, fb_bytes);
-    if (e != hipSuccess) return fail(SHT_ERR_ALLOC, std::string("hipMalloc scratch: ") + hipGetErrorString(e));
-    A.push_back(pl->fbuf);
+    pl->fb_bytes = (fb_bytes + 255) & ~(size_t)255;
 
-    long long bytes = (long long)fb_bytes;
-    bytes += split ? 2 * 2 * (syn_total + ana_total) + 4 * (syn_stotal + ana_stotal) : 4 * (syn_total + ana_total);
+    long long bytes = (long long)fb_bytes + bytes_dft;
+    bytes += split ? 2LL * sht::split_parts(mode) * (syn_total + ana_total) + 4 * (syn_stotal + ana_stotal)
+                   : 4 * (syn_total + ana_total);
     bytes += 8 * tw_total + 16LL * g.nharm + 64LL * n + 48LL * mmax;
     pl->device_bytes = bytes;
     return SHT_OK;
@@ -783,7 +935,8 @@ int sht_plan_sizes(const sht_desc *desc, sht_sizes *sizes) {
         lam += 2LL * ((J + 15) / 16) * ((Le + 31) / 32) * 512;
         lam += 2LL * ((L + adeg - 1) / adeg) * (adeg / 32) * ((J + 31) / 32) * 512;
     }
-    sizes->device_bytes = lam * 4 + g.fb_rows * (long long)desc->nlayers * 16;
+    const int lam_bytes = desc->precision == SHT_PREC_FP32 ? 4 : 2 * sht::split_parts(desc->precision);
+    sizes->device_bytes = lam * lam_bytes + g.fb_rows * (long long)desc->nlayers * 16;
     return SHT_OK;
 }
 
@@ -847,14 +1000,14 @@ static int build_matrices(sht_plan *pl) {
         work = std::max(work, (long long)sht::dense_split(np, 2 * nh, K).s * K * np);
         work = std::max(work, (long long)sht::dense_split(2 * nh, np, K).s * K * 2 * nh);
     }
-    if ((rc = dev_upload(pl->allocs, std::vector<float>((size_t)work, 0.f), &d))) return rc;
-    pl->dense_work = const_cast<float *>(d);
+    pl->dense_bytes = ((size_t)work * 4 + 255) & ~(size_t)255;  // in each stream's scratch
     pl->device_bytes += (long long)bytes + np * 4 + work * 4;
     pl->matrix = true;
     return SHT_OK;
 }
 
-int sht_plan_create(const sht_desc *desc, sht_plan **plan) {
+// A regular plan, or with matrix the dense one, with the null stream's scratch allocated.
+static int create_plan(const sht_desc *desc, bool matrix, sht_plan **plan) {
     if (!plan) return fail(SHT_ERR_ARG, "null plan pointer");
     *plan = nullptr;
     sht_plan *pl = new (std::nothrow) sht_plan();
@@ -873,22 +1026,16 @@ int sht_plan_create(const sht_desc *desc, sht_plan **plan) {
     }
     DeviceGuard guard(desc->device);
     rc = build_plan(pl);
+    if (!rc && matrix) rc = build_matrices(pl);
+    if (!rc && !stream_scratch(pl, nullptr)) rc = SHT_ERR_ALLOC;
     if (rc) { free_plan(pl); return rc; }
     *plan = pl;
     return SHT_OK;
 }
 
-int sht_plan_create_matrix(const sht_desc *desc, sht_plan **plan) {
-    int rc = sht_plan_create(desc, plan);
-    if (rc) return rc;
-    DeviceGuard guard(desc->device);
-    rc = build_matrices(*plan);
-    if (rc) {
-        free_plan(*plan);
-        *plan = nullptr;
-    }
-    return rc;
-}
+int sht_plan_create(const sht_desc *desc, sht_plan **plan) { return create_plan(desc, false, plan); }
+
+int sht_plan_create_matrix(const sht_desc *desc, sht_plan **plan) { return create_plan(desc, true, plan); }
 
 int sht_plan_destroy(sht_plan *plan) {
     if (!plan) return SHT_OK;
@@ -934,39 +1081,73 @@ struct LapOp {
     float sign = 1.f, scale = 1.f;
 };
 
+// Launches one stage of a single call (c: its arguments, K columns) on the plan's chosen
+// launch orders, bracketed by profiling events when profiling is on.
+static hipError_t launch_plan_stage(sht_plan *plan, int stage, const CallArgs &c, void *stream) {
+    const int K = c.K;
+    const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
+    const int gx = plan->leg_xcd;
+    static const int kernel_id[4] = {SHT_KERNEL_LEGENDRE_SYNTHESIS, SHT_KERNEL_FFT_SYNTHESIS, SHT_KERNEL_FFT_ANALYSIS,
+                                     SHT_KERNEL_LEGENDRE_ANALYSIS};
+    ProfScope ps(plan, kernel_id[stage], stream);
+    switch (stage) {
+        case sht::kLegendreSynthesis:
+            return launch_stage(sht::legendre_kernel(stage, plan->d.precision, false), dim3(plan->nsyn[gx][cgs - 1]),
+                                0, plan->dP, c, nullptr, plan->syn_tiles[gx][cgs - 1], stream);
+        case sht::kFFTSynthesis:
+        case sht::kFFTAnalysis: {
+            if (plan->fourier != sht::kFourierRadix) {  // ring DFT on MFMA: blocks (ring rank, column group)
+                const bool f16 = plan->fourier == sht::kFourierDft16;
+                const int cg = plan->dft_cg;
+                const bool syn = stage == sht::kFFTSynthesis;
+                return launch_stage(sht::dft_kernel(stage, f16, cg, false, syn && c.accumulate != 0),
+                                    dim3(plan->g.n * ((K + cg - 1) / cg)), plan->dft_smem[syn][f16][cg == 8],
+                                    plan->dP, c, nullptr, nullptr, stream);
+            }
+            if (stage == sht::kFFTSynthesis)
+                return launch_stage(sht::fft_kernel(stage, false, c.accumulate != 0),
+                                    dim3(fft_blocks(plan->g.n, K, c.paired != 0)), plan->P.fft_smem, plan->dP, c,
+                                    nullptr, nullptr, stream);
+            return launch_stage(sht::fft_kernel(stage, false), dim3(fft_blocks(plan->g.n, K, false)), plan->P.fft_smem,
+                                plan->dP, c, nullptr, nullptr, stream);
+        }
+        default:
+            return launch_stage(sht::legendre_kernel(stage, plan->d.precision, false), dim3(plan->nana[gx][cgs - 1]),
+                                0, plan->dP, c, nullptr, plan->ana_tiles[gx][cgs - 1], stream);
+    }
+}
+
+// Scratch of a call: the caller's (sized for the plan), else the stream's own.
+static void *call_scratch(sht_plan *plan, void *scratch, void *stream) {
+    return scratch ? scratch : stream_scratch(plan, stream);
+}
+
 // Spectral -> grid stages (Legendre synthesis, FFT synthesis); adj = 2 and accumulate turn
-// them into the pullback of the analysis; op fuses a Laplacian of the input.
+// them into the pullback of the analysis; op fuses a Laplacian of the input; scratch (if
+// not null) replaces the stream's scratch.
 static int run_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *field, int64_t ld_grid, int32_t K,
-                         int32_t unscale_coslat, int adj, int accumulate, void *stream, LapOp op = LapOp()) {
+                         int32_t unscale_coslat, int adj, int accumulate, void *stream, LapOp op = LapOp(),
+                         void *scratch = nullptr) {
     int rc = check_call(plan, coeffs, field, K);
     if (rc) return rc;
     if (ld_spec < plan->g.nharm) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_spec < nharm");
     if (ld_grid < plan->g.npoints) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_grid < npoints");
     DeviceGuard guard(plan->d.device);
-    const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
+    void *ws = call_scratch(plan, scratch, stream);
+    if (!ws) return SHT_ERR_ALLOC;
     if (plan->matrix && adj == 0 && !accumulate && op.lap == 0) {  // field = B_int a_int, one GEMM (rows / cos(lat))
+        float *work = static_cast<float *>(ws);  // the GEMM needs no Fourier buffer
         rc = sht::launch_dense_gemm(plan->mat_syn, plan->g.npoints, 2 * plan->g.nharm,
                                     static_cast<const float *>(coeffs), 2 * ld_spec, K, static_cast<float *>(field),
-                                    ld_grid, unscale_coslat ? plan->pt_coslat_inv : nullptr, plan->dense_work, stream);
+                                    ld_grid, unscale_coslat ? plan->pt_coslat_inv : nullptr, work, stream);
         if (rc) return fail(SHT_ERR_HIP, std::string("dense synthesis launch: ") + hipGetErrorString((hipError_t)rc));
         return SHT_OK;
     }
-    const CallArgs c{0, K, unscale_coslat ? 1 : 0, plan->fft_pair, coeffs, field, ld_spec, ld_grid, plan->fbuf,
+    const CallArgs c{0, K, unscale_coslat ? 1 : 0, plan->fft_pair, coeffs, field, ld_spec, ld_grid, ws,
                      adj, accumulate ? 1 : 0, op.lap, op.sign, op.scale};
-    hipError_t e;
-    {
-        ProfScope ps(plan, SHT_KERNEL_LEGENDRE_SYNTHESIS, stream);
-        const int gx = plan->leg_xcd;
-        e = launch_stage(sht::legendre_kernel(sht::kLegendreSynthesis, plan->d.precision, false),
-                         dim3(plan->nsyn[gx][cgs - 1]), 0, plan->dP, c, nullptr, plan->syn_tiles[gx][cgs - 1], stream);
-    }
+    hipError_t e = launch_plan_stage(plan, sht::kLegendreSynthesis, c, stream);
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("legendre synthesis launch: ") + hipGetErrorString(e));
-    {
-        ProfScope ps(plan, SHT_KERNEL_FFT_SYNTHESIS, stream);
-        e = launch_stage(sht::fft_kernel(sht::kFFTSynthesis, false, accumulate != 0),
-                         dim3(fft_blocks(plan->g.n, K, plan->fft_pair != 0)),
-                         plan->P.fft_smem, plan->dP, c, nullptr, nullptr, stream);
-    }
+    e = launch_plan_stage(plan, sht::kFFTSynthesis, c, stream);
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("fft synthesis launch: ") + hipGetErrorString(e));
     return SHT_OK;
 }
@@ -974,35 +1155,28 @@ static int run_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, vo
 // Grid -> spectral stages (FFT analysis, Legendre analysis); adj = 1 and accumulate turn
 // them into the pullback of the synthesis; op fuses a Laplacian of the output.
 static int run_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void *coeffs, int64_t ld_spec, int32_t K,
-                        int32_t unscale_coslat, int adj, int accumulate, void *stream, LapOp op = LapOp()) {
+                        int32_t unscale_coslat, int adj, int accumulate, void *stream, LapOp op = LapOp(),
+                        void *scratch = nullptr) {
     int rc = check_call(plan, field, coeffs, K);
     if (rc) return rc;
     if (ld_spec < plan->g.nharm) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_spec < nharm");
     if (ld_grid < plan->g.npoints) return fail(SHT_ERR_DIM, "DimensionMismatch: ld_grid < npoints");
     DeviceGuard guard(plan->d.device);
-    const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
+    void *ws = call_scratch(plan, scratch, stream);
+    if (!ws) return SHT_ERR_ALLOC;
     if (plan->matrix && adj == 0 && !accumulate && op.lap == 0) {  // a_int = F_int f, one GEMM
+        float *work = static_cast<float *>(ws);  // the GEMM needs no Fourier buffer
         rc = sht::launch_dense_gemm(plan->mat_ana, 2 * plan->g.nharm, plan->g.npoints,
                                     static_cast<const float *>(field), ld_grid, K, static_cast<float *>(coeffs),
-                                    2 * ld_spec, nullptr, plan->dense_work, stream);
+                                    2 * ld_spec, nullptr, work, stream);
         if (rc) return fail(SHT_ERR_HIP, std::string("dense analysis launch: ") + hipGetErrorString((hipError_t)rc));
         return SHT_OK;
     }
-    const CallArgs c{0,           K,          unscale_coslat ? 1 : 0, 0,       field,   coeffs, ld_grid, ld_spec,
-                     plan->fbuf,  adj,        accumulate ? 1 : 0,     op.lap, op.sign, op.scale};
-    hipError_t e;
-    {
-        ProfScope ps(plan, SHT_KERNEL_FFT_ANALYSIS, stream);
-        e = launch_stage(sht::fft_kernel(sht::kFFTAnalysis, false), dim3(fft_blocks(plan->g.n, K, false)),
-                         plan->P.fft_smem, plan->dP, c, nullptr, nullptr, stream);
-    }
+    const CallArgs c{0,  K,   unscale_coslat ? 1 : 0, 0,      field,   coeffs, ld_grid, ld_spec,
+                     ws, adj, accumulate ? 1 : 0,     op.lap, op.sign, op.scale};
+    hipError_t e = launch_plan_stage(plan, sht::kFFTAnalysis, c, stream);
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("fft analysis launch: ") + hipGetErrorString(e));
-    {
-        ProfScope ps(plan, SHT_KERNEL_LEGENDRE_ANALYSIS, stream);
-        const int gx = plan->leg_xcd;
-        e = launch_stage(sht::legendre_kernel(sht::kLegendreAnalysis, plan->d.precision, false),
-                         dim3(plan->nana[gx][cgs - 1]), 0, plan->dP, c, nullptr, plan->ana_tiles[gx][cgs - 1], stream);
-    }
+    e = launch_plan_stage(plan, sht::kLegendreAnalysis, c, stream);
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("legendre analysis launch: ") + hipGetErrorString(e));
     return SHT_OK;
 }
@@ -1026,6 +1200,27 @@ int sht_grid2spec_pullback(sht_plan *plan, const void *coeffs_bar, int64_t ld_sp
                            int32_t K, int32_t accumulate, void *stream) {
     return run_synthesis(plan, coeffs_bar, ld_spec, field_bar, ld_grid, K, 0, 2, accumulate, stream);
 }
+
+}  // extern "C"
+
+size_t sht::plan_scratch_bytes(const sht_plan *plan, int K) {
+    return plan->matrix ? plan->dense_bytes : (size_t)plan->g.fb_rows * (size_t)K * 16;
+}
+
+int sht::plan_device(const sht_plan *plan) { return plan->d.device; }
+
+int sht::transform_with_scratch(sht_plan *plan, int direction, const void *in, int64_t ld_in, void *out,
+                                int64_t ld_out, int32_t K, int32_t unscale_coslat, void *scratch, void *stream) {
+    if (direction == SHT_DIR_SYNTHESIS)
+        return run_synthesis(plan, in, ld_in, out, ld_out, K, unscale_coslat, 0, 0, stream, LapOp(), scratch);
+    if (direction == SHT_DIR_ANALYSIS) {
+        if (unscale_coslat) return fail(SHT_ERR_ARG, "unscale_coslat applies to spectral->grid transforms only");
+        return run_analysis(plan, in, ld_in, out, ld_out, K, 0, 0, 0, stream, LapOp(), scratch);
+    }
+    return fail(SHT_ERR_ARG, "direction must be SHT_DIR_SYNTHESIS or SHT_DIR_ANALYSIS");
+}
+
+extern "C" {
 
 // The Laplacian's factors on a sphere of the given radius (as sht_laplace: 1/R² for ∇², R²
 // for ∇⁻²), or an error status.
@@ -1112,6 +1307,39 @@ int sht_gradient(sht_plan *plan, const void *p, void *dpdx, void *dpdy, int64_t 
     return SHT_OK;
 }
 
+int sht_time_stage(sht_plan *plan, int32_t stage, void *coeffs, int64_t ld_spec, void *field, int64_t ld_grid, int32_t K,
+                   int32_t reps, void *stream, double *avg_ms) {
+    if (!avg_ms || reps < 1 || stage < 0 || stage > 3) return fail(SHT_ERR_ARG, "bad arguments");
+    int rc = check_call(plan, coeffs, field, K);
+    if (rc) return rc;
+    if (plan->matrix) return fail(SHT_ERR_UNSUPPORTED, "a dense plan has no transform stages");
+    if (ld_spec < plan->g.nharm || ld_grid < plan->g.npoints) return fail(SHT_ERR_DIM, "DimensionMismatch: stride");
+    DeviceGuard guard(plan->d.device);
+    void *ws = stream_scratch(plan, stream);
+    if (!ws) return SHT_ERR_ALLOC;
+    const bool spec_in = stage == sht::kLegendreSynthesis || stage == sht::kFFTSynthesis;
+    const CallArgs c{0, K, 0, spec_in ? plan->fft_pair : 0, spec_in ? coeffs : field, spec_in ? field : coeffs,
+                     spec_in ? ld_spec : ld_grid, spec_in ? ld_grid : ld_spec, ws};
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return fail(SHT_ERR_HIP, "hipEventCreate failed");
+    }
+    hipError_t e = launch_plan_stage(plan, stage, c, stream);  // untimed: first-launch costs
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int i = 0; i < reps && e == hipSuccess; ++i) e = launch_plan_stage(plan, stage, c, stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("stage timing: ") + hipGetErrorString(e));
+    *avg_ms = (double)ms / reps;
+    return SHT_OK;
+}
+
 int sht_profile_begin(sht_plan *plan, int32_t max_calls) {
     if (!plan || max_calls < 1) return fail(SHT_ERR_ARG, "bad arguments");
     DeviceGuard guard(plan->d.device);
@@ -1152,25 +1380,15 @@ int sht_profile_end(sht_plan *plan, double *avg_ms, int32_t *counts) {
 }
 
 namespace {
-// Uploads the FFT tables of a factorisation policy into the plan's device arrays.
-int apply_fft_policy(sht_plan *plan, const FFTPolicy &pol, void *stream) {
-    std::vector<int> fac, nfac, order;
-    std::vector<double> cost;
-    int rc = fft_tables(plan->g, pol, fac, nfac, cost, order);
-    if (rc) return rc;
-    const std::vector<sht::FFTPass> passes = pass_table(plan->g, fac, nfac);
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    if (hipStreamSynchronize(st) != hipSuccess ||
-        hipMemcpy(const_cast<sht::FFTPass *>(plan->P.passes), passes.data(), passes.size() * sizeof(sht::FFTPass),
-                  hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(const_cast<int *>(plan->P.nfac), nfac.data(), nfac.size() * sizeof(int), hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        hipMemcpy(const_cast<int *>(plan->P.ring_order), order.data(), order.size() * sizeof(int),
-                  hipMemcpyHostToDevice) != hipSuccess)
-        return fail(SHT_ERR_HIP, "uploading FFT tables failed");
-    plan->fft_policy = pol;
-    plan->h_ring_order = order;
-    plan->h_ring_cost = cost;
+// Chooses the FFT tables of a factorisation policy (all are on the device already: only
+// the plan's host state changes, launches already queued keep the tables they were given).
+int apply_fft_policy(sht_plan *plan, const FFTPolicy &pol) {
+    const int q = policy_index(pol);
+    if (!plan->pol_ok[q])
+        return fail(SHT_ERR_UNSUPPORTED, "the FFT factorisation policy has no kernel for some ring length");
+    plan->pol = q;
+    plan->P = plan->Pv[q];
+    plan->dP = plan->dPv + q;
     return SHT_OK;
 }
 
@@ -1184,10 +1402,104 @@ struct TuneKey {
 struct TuneChoice {
     FFTPolicy pol;
     int leg_xcd, fft_pair;
+    int fourier, dft_cg;
 };
 std::map<TuneKey, TuneChoice> g_tune_cache;
 std::mutex g_tune_mutex;
+
+// The autotuner also times the ring DFT on MFMA.
+constexpr bool kAutotuneDft = false;
+
+// Applies a tuning choice (host state only).
+int apply_choice(sht_plan *plan, const TuneChoice &c) {
+    if (c.fourier == sht::kFourierDft16 && plan->d.precision == SHT_PREC_FP32)
+        return fail(SHT_ERR_UNSUPPORTED, "the FP16 ring DFT needs a split-precision plan");
+    const int rc = apply_fft_policy(plan, c.pol);
+    if (rc) return rc;
+    plan->leg_xcd = c.leg_xcd;
+    plan->fft_pair = c.fft_pair;
+    plan->fourier = c.fourier;
+    plan->dft_cg = c.dft_cg;
+    return SHT_OK;
+}
+
+// Tuning cache file: one flat JSON object of integers per line, keys as below.
+const char *const kTuneFields[] = {"grid", "nlat_half", "lmax", "mmax", "nlayers", "precision", "K",
+                                   "mfma_min", "eights", "odd_first", "leg_xcd", "fft_pair", "fourier", "dft_cg"};
+constexpr int kTuneNFields = 14;
+
+void tune_to_ints(const TuneKey &k, const TuneChoice &c, long long *v) {
+    const long long a[kTuneNFields] = {k.grid, k.nlat_half, k.lmax, k.mmax, k.nlayers, k.precision, k.K,
+                                       c.pol.mfma_min, c.pol.eights, c.pol.odd_first, c.leg_xcd, c.fft_pair,
+                                       c.fourier, c.dft_cg};
+    std::copy(a, a + kTuneNFields, v);
+}
+
+// Parses one line; false if a field is missing or a value out of range.
+bool tune_from_line(const std::string &line, TuneKey &k, TuneChoice &c) {
+    long long v[kTuneNFields];
+    for (int f = 0; f < kTuneNFields; ++f) {
+        const std::string key = std::string("\"") + kTuneFields[f] + "\"";
+        const size_t at = line.find(key);
+        if (at == std::string::npos) return false;
+        size_t q = line.find(':', at + key.size());
+        if (q == std::string::npos) return false;
+        char *end = nullptr;
+        v[f] = std::strtoll(line.c_str() + q + 1, &end, 10);
+        if (end == line.c_str() + q + 1) return false;
+    }
+    k = TuneKey{(int)v[0], (int)v[1], (int)v[2], (int)v[3], (int)v[4], (int)v[5], (int)v[6]};
+    c.pol.mfma_min = (int)v[7];
+    c.pol.eights = v[8] != 0;
+    c.pol.odd_first = v[9] != 0;
+    c.leg_xcd = (int)v[10];
+    c.fft_pair = (int)v[11];
+    c.fourier = (int)v[12];
+    c.dft_cg = (int)v[13];
+    return (c.pol.mfma_min == 11 || c.pol.mfma_min == 17) && c.leg_xcd >= 0 && c.leg_xcd <= 1 && c.fft_pair >= 0 &&
+           c.fft_pair <= 1 && c.fourier >= 0 && c.fourier <= 2 && (c.dft_cg == 4 || c.dft_cg == 8) && k.K >= 1;
+}
 }  // namespace
+
+int sht_tune_cache_save(const char *path) {
+    if (!path) return fail(SHT_ERR_ARG, "null path");
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    FILE *f = std::fopen(path, "w");
+    if (!f) return fail(SHT_ERR_ARG, std::string("cannot write ") + path);
+    for (const auto &kv : g_tune_cache) {
+        long long v[kTuneNFields];
+        tune_to_ints(kv.first, kv.second, v);
+        std::fputc('{', f);
+        for (int i = 0; i < kTuneNFields; ++i) std::fprintf(f, "%s\"%s\": %lld", i ? ", " : "", kTuneFields[i], v[i]);
+        std::fputs("}\n", f);
+    }
+    const bool ok = std::fclose(f) == 0;
+    return ok ? SHT_OK : fail(SHT_ERR_ARG, std::string("cannot write ") + path);
+}
+
+int sht_tune_cache_load(const char *path, int32_t *nloaded) {
+    if (!path) return fail(SHT_ERR_ARG, "null path");
+    FILE *f = std::fopen(path, "r");
+    if (!f) return fail(SHT_ERR_ARG, std::string("cannot read ") + path);
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    int n = 0, bad = 0;
+    char buf[1024];
+    while (std::fgets(buf, sizeof buf, f)) {
+        const std::string line(buf);
+        if (line.find('{') == std::string::npos) continue;
+        TuneKey k;
+        TuneChoice c;
+        if (tune_from_line(line, k, c)) {
+            g_tune_cache[k] = c;
+            ++n;
+        } else {
+            ++bad;
+        }
+    }
+    std::fclose(f);
+    if (nloaded) *nloaded = n;
+    return bad ? fail(SHT_ERR_ARG, std::to_string(bad) + " malformed line(s) in " + path) : SHT_OK;
+}
 
 int sht_plan_autotune(sht_plan *plan, int32_t K, void *stream) {
     if (!plan) return fail(SHT_ERR_ARG, "null plan");
@@ -1198,11 +1510,7 @@ int sht_plan_autotune(sht_plan *plan, int32_t K, void *stream) {
     {
         std::lock_guard<std::mutex> lock(g_tune_mutex);
         auto it = g_tune_cache.find(key);
-        if (it != g_tune_cache.end()) {
-            const int rc = apply_fft_policy(plan, it->second.pol, stream);
-            if (rc) return rc;
-            plan->leg_xcd = it->second.leg_xcd;
-            plan->fft_pair = it->second.fft_pair;
+        if (it != g_tune_cache.end() && apply_choice(plan, it->second) == SHT_OK) {
             plan->tuned = 1;
             return SHT_OK;
         }
@@ -1238,39 +1546,45 @@ int sht_plan_autotune(sht_plan *plan, int32_t K, void *stream) {
         }
         return SHT_OK;
     };
-    TuneChoice best{plan->fft_policy, plan->leg_xcd, plan->fft_pair};
+    TuneChoice best{policy_of(plan->pol), plan->leg_xcd, plan->fft_pair, plan->fourier, plan->dft_cg};
     if (!rc) {
+        // Fourier stage: each factorisation policy of the LDS FFT, then the ring DFT on MFMA
+        // (FP32, and FP16 hi/lo on split plans) with 4 or 8 columns per workgroup
         float best_ms = 1e30f;
-        for (int mmin : {17, 11})
-            for (int eights = 1; eights >= 0; --eights)
-                for (int odd_first = 0; odd_first < 2 && !rc; ++odd_first) {
-                    FFTPolicy pol;
-                    pol.mfma_min = mmin;
-                    pol.eights = eights != 0;
-                    pol.odd_first = odd_first != 0;
-                    float ms = 0.f;
-                    if ((rc = apply_fft_policy(plan, pol, stream)) || (rc = round_trip_ms(ms))) break;
-                    if (ms < best_ms) {
-                        best_ms = ms;
-                        best.pol = pol;
-                    }
-                }
-        if (!rc) rc = apply_fft_policy(plan, best.pol, stream);
+        std::vector<TuneChoice> cands;
+        for (int q = 0; q < kPolicies; ++q)
+            if (plan->pol_ok[q]) cands.push_back({policy_of(q), best.leg_xcd, best.fft_pair, sht::kFourierRadix, 4});
+        if (kAutotuneDft)
+            for (int kind : {sht::kFourierDft32, sht::kFourierDft16})
+                for (int cg : {4, 8})
+                    if (kind == sht::kFourierDft32 || plan->d.precision != SHT_PREC_FP32)
+                        cands.push_back({best.pol, best.leg_xcd, best.fft_pair, kind, cg});
+        for (const TuneChoice &c : cands) {
+            float ms = 0.f;
+            if ((rc = apply_choice(plan, c)) || (rc = round_trip_ms(ms))) break;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+        if (!rc) rc = apply_choice(plan, best);
         best_ms = 1e30f;
+        TuneChoice b2 = best;
         for (int xcd = 1; xcd >= 0 && !rc; --xcd)
             for (int pair = 1; pair >= 0 && !rc; --pair) {
-                plan->leg_xcd = xcd;
-                plan->fft_pair = pair;
+                if (pair == 0 && best.fourier != sht::kFourierRadix) continue;  // the DFT has one order
+                TuneChoice c = best;
+                c.leg_xcd = xcd;
+                c.fft_pair = pair;
                 float ms = 0.f;
-                if ((rc = round_trip_ms(ms))) break;
+                if ((rc = apply_choice(plan, c)) || (rc = round_trip_ms(ms))) break;
                 if (ms < best_ms) {
                     best_ms = ms;
-                    best.leg_xcd = xcd;
-                    best.fft_pair = pair;
+                    b2 = c;
                 }
             }
-        plan->leg_xcd = best.leg_xcd;
-        plan->fft_pair = best.fft_pair;
+        best = b2;
+        if (!rc) rc = apply_choice(plan, best);
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -1287,28 +1601,26 @@ int sht_plan_autotune(sht_plan *plan, int32_t K, void *stream) {
 int sht_plan_set_tuning(sht_plan *plan, const int32_t *in) {
     if (!plan || !in) return fail(SHT_ERR_ARG, "bad arguments");
     if ((in[0] != 11 && in[0] != 17) || in[1] < 0 || in[1] > 1 || in[2] < 0 || in[2] > 1 || in[3] < 0 ||
-        in[3] > 1 || in[4] < 0 || in[4] > 1)
+        in[3] > 1 || in[4] < 0 || in[4] > 1 || in[5] < 0 || in[5] > 2 || (in[6] != 4 && in[6] != 8))
         return fail(SHT_ERR_ARG, "tuning values out of range");
-    DeviceGuard guard(plan->d.device);
     FFTPolicy pol;
     pol.mfma_min = in[0];
     pol.eights = in[1] != 0;
     pol.odd_first = in[2] != 0;
-    const int rc = apply_fft_policy(plan, pol, nullptr);
-    if (rc) return rc;
-    plan->leg_xcd = in[3];
-    plan->fft_pair = in[4];
-    return SHT_OK;
+    return apply_choice(plan, TuneChoice{pol, in[3], in[4], in[5], in[6]});
 }
 
 int sht_plan_tuning(const sht_plan *plan, int32_t *out) {
     if (!plan || !out) return fail(SHT_ERR_ARG, "bad arguments");
     out[0] = plan->tuned;
-    out[1] = plan->fft_policy.mfma_min;
-    out[2] = plan->fft_policy.eights ? 1 : 0;
-    out[3] = plan->fft_policy.odd_first ? 1 : 0;
+    const FFTPolicy pol = policy_of(plan->pol);
+    out[1] = pol.mfma_min;
+    out[2] = pol.eights ? 1 : 0;
+    out[3] = pol.odd_first ? 1 : 0;
     out[4] = plan->leg_xcd;
     out[5] = plan->fft_pair;
+    out[6] = plan->fourier;
+    out[7] = plan->dft_cg;
     return SHT_OK;
 }
 
@@ -1394,12 +1706,12 @@ int build_batch(sht_batch *b, const sht_batch_item *items, int n, int flags) {
         const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
         const int kgs = (K + pl->P.kg - 1) / pl->P.kg;
         auto add_fft = [&](int st, int call) {
-            for (int j : pl->h_ring_order)
+            for (int j : pl->h_ring_order_v[pl->pol])
                 for (int q = 0; q < kgs; q += 2) {
                     LaunchUnit u{0.0, {}};
                     for (int h = q; h < std::min(q + 2, kgs); ++h) {
                         u.tiles.push_back({call, j, h, 0});
-                        u.cost += pl->h_ring_cost[j];
+                        u.cost += pl->h_ring_cost_v[pl->pol][j];
                     }
                     st_units[st].push_back(u);
                 }

@@ -7,6 +7,10 @@
 # GPU{ROCBackend}.  The SpectralTransform type is unchanged; the engine's plan for each
 # transform lives in a global IdDict keyed by the transform (the same pattern the
 # reference uses for its per-transform GPU caches, ext/gpu_graphs_common.jl:116-121).
+# SpectralTransform is immutable, so no finalizer can be attached to it: plans live until
+# free_plans() (also registered with atexit) destroys them.  libsht keeps a Fourier scratch
+# per (plan, stream), so tasks on different AMDGPU streams may transform with one S at once;
+# the reference's ScratchMemory argument is therefore not needed and is ignored.
 #
 # Wiring into SpeedyTransforms/Project.toml:
 #   [weakdeps]    AMDGPU = "21141c5a-9bdb-4563-92ae-f87d6854732e"
@@ -79,13 +83,23 @@ function plan(S::SpectralTransform)
             p = Ref{Ptr{Cvoid}}(C_NULL)
             rc = ccall((:sht_plan_create, libsht), Cint, (Ref{ShtDesc}, Ref{Ptr{Cvoid}}), desc, p)
             rc == SHT_OK || error("libsht plan: " * last_error())
-            finalizer(S) do _
-                ccall((:sht_plan_destroy, libsht), Cint, (Ptr{Cvoid},), p[])
-            end
             p[]
         end
     end
 end
+
+"""Destroys every plan made so far (waits for the device first); later transforms make new ones."""
+function free_plans()
+    lock(PLANS_LOCK) do
+        for p in values(PLANS)
+            ccall((:sht_plan_destroy, libsht), Cint, (Ptr{Cvoid},), p)
+        end
+        empty!(PLANS)
+    end
+    return nothing
+end
+
+__init__() = atexit(free_plans)
 
 stream_handle() = AMDGPU.stream().stream
 ncols(A) = ndims(A) == 1 ? 1 : size(A, 2)
@@ -103,6 +117,7 @@ function _transform_grid!(field::AbstractField, coeffs::LowerTriangularArray,
     f = field.data
     eltype(f) == Float32 || throw(ArgumentError("field must be Float32 like the transform"))
     K = ncols(f)
+    ncols(c) == K || throw(DimensionMismatch(S, coeffs))   # the kernels read K columns of coeffs
     GC.@preserve c f begin
         rc = ccall((:sht_synthesis, libsht), Cint,
                    (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Int32, Int32, Ptr{Cvoid}),
@@ -121,6 +136,7 @@ function _transform_spec!(coeffs::LowerTriangularArray, field::AbstractField,
     c = coeffs.data
     f = field.data
     K = ncols(f)
+    ncols(c) == K || throw(DimensionMismatch(S, coeffs))   # the kernels write K columns of coeffs
     GC.@preserve c f begin
         rc = ccall((:sht_analysis, libsht), Cint,
                    (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Int32, Ptr{Cvoid}),

@@ -69,17 +69,24 @@ class SpectralTransform:
         check(lib().sht_plan_autotune(self._plan, K, self._stream()), "sht_plan_autotune")
         return self.tuning()
 
-    def set_tuning(self, mfma_min=17, eights=True, odd_first=False, legendre_xcd_groups=True, fft_pairs=True):
-        """Pins the FFT factorisation policy and launch orders (sht_plan_set_tuning)."""
-        v = (C.c_int32 * 5)(int(mfma_min), int(bool(eights)), int(bool(odd_first)), int(bool(legendre_xcd_groups)),
-                            int(bool(fft_pairs)))
+    FOURIER = {"radix": 0, "dft32": 1, "dft16": 2}
+
+    def set_tuning(self, mfma_min=17, eights=True, odd_first=False, legendre_xcd_groups=True, fft_pairs=True,
+                   fourier="radix", dft_cols=4):
+        """Pins the Fourier kernel (the LDS FFT with its factorisation policy, or the ring
+        DFT on MFMA with dft_cols columns per workgroup) and the launch orders
+        (sht_plan_set_tuning)."""
+        v = (C.c_int32 * 7)(int(mfma_min), int(bool(eights)), int(bool(odd_first)), int(bool(legendre_xcd_groups)),
+                            int(bool(fft_pairs)), self.FOURIER[fourier], int(dft_cols))
         check(lib().sht_plan_set_tuning(self._plan, v), "sht_plan_set_tuning")
 
     def tuning(self) -> dict:
-        out = (C.c_int32 * 6)()
+        out = (C.c_int32 * 8)()
         check(lib().sht_plan_tuning(self._plan, out), "sht_plan_tuning")
+        kinds = {v: k for k, v in self.FOURIER.items()}
         return {"tuned": bool(out[0]), "mfma_min": out[1], "eights": bool(out[2]), "odd_first": bool(out[3]),
-                "legendre_xcd_groups": bool(out[4]), "fft_pairs": bool(out[5])}
+                "legendre_xcd_groups": bool(out[4]), "fft_pairs": bool(out[5]), "fourier": kinds[out[6]],
+                "dft_cols": out[7]}
 
     def close(self):
         if getattr(self, "_plan", None) and self._plan.value:
@@ -359,25 +366,28 @@ class SpectralBatch:
             rest = list(it[3:])
             mid = rest.pop(0) if rest and isinstance(rest[0], Field) else None
             unscale = bool(rest[0]) if rest else False
+            Ks = []
             if mid is not None:
                 direction = _lib.SHT_DIR_ROUNDTRIP
                 a, K, ld_in = S._check_tensor(inp.data, S.spectrum.nharm, torch.complex64, "coeffs")
-                m2, _, ld_mid = S._check_tensor(mid.data, S.grid.npoints, torch.float32, "field")
-                o, _, ld_out = S._check_tensor(out.data, S.spectrum.nharm, torch.complex64, "coeffs")
-                checks = (inp, mid, out)
+                m2, Km, ld_mid = S._check_tensor(mid.data, S.grid.npoints, torch.float32, "field")
+                o, Ko, ld_out = S._check_tensor(out.data, S.spectrum.nharm, torch.complex64, "coeffs")
+                checks, Ks = (inp, mid, out), [Km, Ko]
             elif isinstance(inp, LowerTriangularArray):
                 direction = _lib.SHT_DIR_SYNTHESIS
                 a, K, ld_in = S._check_tensor(inp.data, S.spectrum.nharm, torch.complex64, "coeffs")
-                o, _, ld_out = S._check_tensor(out.data, S.grid.npoints, torch.float32, "field")
-                m2, ld_mid, checks = None, 0, (inp, out)
+                o, Ko, ld_out = S._check_tensor(out.data, S.grid.npoints, torch.float32, "field")
+                m2, ld_mid, checks, Ks = None, 0, (inp, out), [Ko]
             else:
                 direction = _lib.SHT_DIR_ANALYSIS
                 a, K, ld_in = S._check_tensor(inp.data, S.grid.npoints, torch.float32, "field")
-                o, _, ld_out = S._check_tensor(out.data, S.spectrum.nharm, torch.complex64, "coeffs")
-                m2, ld_mid, checks = None, 0, (inp, out)
+                o, Ko, ld_out = S._check_tensor(out.data, S.spectrum.nharm, torch.complex64, "coeffs")
+                m2, ld_mid, checks, Ks = None, 0, (inp, out), [Ko]
             for x in checks:
                 if not S.ismatching(x):
                     raise _mismatch(S, x)
+            if any(k != K for k in Ks):  # the kernels write K columns of every output
+                raise DimensionMismatch(f"batch item {i}: input K={K}, output/grid K={Ks} differ")
             arr[i] = _lib.ShtBatchItem(S._plan.value, direction, K, int(unscale), 0, a.data_ptr(), ld_in,
                                        m2.data_ptr() if m2 is not None else None, ld_mid, o.data_ptr(), ld_out)
             self._keep.append((S, a, m2, o))

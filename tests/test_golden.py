@@ -37,7 +37,7 @@ def test_engine_matches_fixture(sw, gpu, fix, precision):
     import torch
     sp, grid = sw.Spectrum(33, 32), sw.OctahedralGaussianGrid(24)
     S = sw.SpectralTransform(sp, grid, nlayers=8, precision=precision)
-    tol = 2e-6 if precision != "bf16x3" else 2e-5
+    tol = 2e-6
     for seed in (0, 1):
         c = sw.LowerTriangularArray(torch.from_numpy(fix[f"coeffs_{seed}"]).to(torch.complex64).cuda(), sp)
         f = sw.transform(c, S)

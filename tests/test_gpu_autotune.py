@@ -21,17 +21,20 @@ def fresh_plan(sw, name, precision="fp32"):
     return sw.SpectralTransform(sw.Spectrum(lmax, mmax), grid, nlayers=layers * fields, precision=precision)
 
 
-@pytest.mark.parametrize("name", ["T127", "T31full"])
-def test_every_pinned_configuration_matches_oracle(sw, gpu, name):
+@pytest.mark.parametrize("name, precision", [("T127", "fp32"), ("T31full", "fp32"), ("T255", "fp32"),
+                                             ("T255", "fp16x3")])
+def test_every_pinned_configuration_matches_oracle(sw, gpu, name, precision):
+    """At T255 this covers the MFMA prime pass with more row tiles than waves (primes of
+    131 and above occur only on its rings) under every policy."""
     o, a, layers, fields = case(sw, name, seed=1)
     f_ref = o.synthesis(a)
     b_ref = o.analysis(f_ref)
-    S = fresh_plan(sw, name)
+    S = fresh_plan(sw, name, precision)
     coeffs = to_dev(sw, a, S.spectrum)
     fin = sw.Field(torch.from_numpy(f_ref.astype(np.float32)).cuda(), S.grid)
     for (mmin, eights, odd_first), xcd, pair in itertools.product(POLICIES, [True, False], [True, False]):
         S.set_tuning(mmin, eights, odd_first, xcd, pair)
-        print(f"{name} policy {(mmin, eights, odd_first, xcd, pair)}", flush=True)
+        print(f"{name} {precision} policy {(mmin, eights, odd_first, xcd, pair)}", flush=True)
         f = sw.transform(coeffs, S).data.cpu().numpy()
         b = sw.transform(fin, S).data.cpu().numpy()
         assert rel(f, f_ref) < 1e-5

@@ -75,6 +75,22 @@ def test_graph_replays_follow_inputs(sw, gpu):
     B.close()
 
 
+def test_batch_rejects_mismatched_layer_counts(sw, gpu):
+    """Every output (and a round trip's grid) must hold as many layers as the input: the
+    kernels write K columns of each, so a smaller one raises before anything launches."""
+    S = plan(sw, 31)
+    c = spec(sw, S, 4, seed=5)
+    small_f = sw.Field(torch.zeros((1, S.grid.npoints), device="cuda"), S.grid)
+    with pytest.raises(sw.DimensionMismatch):
+        sw.SpectralBatch([(S, c, small_f)])
+    mid = sw.Field(torch.zeros((4, S.grid.npoints), device="cuda"), S.grid)
+    small_c = sw.LowerTriangularArray.zeros(S.spectrum, 2, device="cuda")
+    with pytest.raises(sw.DimensionMismatch):
+        sw.SpectralBatch([(S, c, small_c, mid)])
+    with pytest.raises(sw.DimensionMismatch):
+        sw.SpectralBatch([(S, c, sw.LowerTriangularArray.zeros(S.spectrum, 4, device="cuda"), small_f)])
+
+
 def test_forced_graph_failure_falls_back(sw, gpu):
     S = plan(sw, 31)
     c = spec(sw, S, 4, seed=3)

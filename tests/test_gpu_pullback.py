@@ -43,13 +43,17 @@ def test_spec2grid_pullback_vs_oracle(sw, gpu, name, precision, unscale):
     assert rel(cbar.data.cpu().numpy().astype(np.complex128), ref) < TOL[precision]
 
 
-@pytest.mark.parametrize("name", ["T31", "T127", "T31full"])
+@pytest.mark.parametrize("name", ["T31", "T127", "T255", "T31full"])
 @pytest.mark.parametrize("precision", ["fp32", "fp16x3"])
-def test_grid2spec_pullback_vs_oracle(sw, gpu, name, precision):
+@pytest.mark.parametrize("kcols", [None, 5])
+def test_grid2spec_pullback_vs_oracle(sw, gpu, name, precision, kcols):
+    """kcols=5: a partial last column group in the accumulating FFT synthesis."""
     o, a, layers, fields = case(sw, name, seed=12)
-    K = layers * fields
+    if kcols:
+        a = a[:kcols]
+    K = a.shape[0]
     S = plan(sw, name, precision)
-    print(f"grid2spec_pullback {name} {precision}", flush=True)
+    print(f"grid2spec_pullback {name} {precision} K={K}", flush=True)
     ref = o.grid2spec_pullback(a)
     cbar = to_dev(sw, a, S.spectrum)
     base = grid_cotangent(sw, S, K, 4) * np.abs(ref).max()  # of the result's size

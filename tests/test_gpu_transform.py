@@ -37,7 +37,7 @@ def case(sw, name, seed=0):
 
 
 # tolerance of each precision mode, relative L2 per field (and per degree band for round trips)
-TOL = {"fp32": 1e-5, "fp16x3": 1e-5, "bf16x3": 1e-4}
+TOL = {"fp32": 1e-5, "fp16x3": 1e-5, "bf16x3": 1e-5}
 
 
 @pytest.mark.parametrize("name", ["T31", "T127", "T255", "T31full"])
@@ -70,7 +70,7 @@ def test_synthesis_analysis_roundtrip_vs_fp64(sw, gpu, name, precision):
     assert np.isfinite(f_gpu).all() and np.isfinite(b_gpu).all()
 
 
-@pytest.mark.parametrize("name", ["T31", "T127"])
+@pytest.mark.parametrize("name", ["T31", "T127", "T255"])
 def test_fp32_vs_cpu_fp32_restatement(sw, gpu, name):
     o, a, layers, fields = case(sw, name, seed=3)
     S = plan(sw, name, "fp32")
@@ -82,8 +82,34 @@ def test_fp32_vs_cpu_fp32_restatement(sw, gpu, name):
     fin = sw.Field(torch.from_numpy(f_cpu).cuda(), S.grid)
     back = sw.transform(fin, S)
     torch.cuda.synchronize()
+    print(f"{name}: FP32 GPU vs FP32 C restatement", flush=True)
     assert rel(field.data.cpu().numpy(), f_cpu) < 1e-5
     assert rel(back.data.cpu().numpy(), b_cpu) < 1e-5
+    # per field and degree band of the analysis, as the round-trip criterion is stated
+    errs = per_field_band_errors(back.data.cpu().numpy().astype(np.complex128), b_cpu.astype(np.complex128),
+                                 o.spectrum.l_of, layers, fields)
+    assert max(errs.values()) < 1e-5, errs
+
+
+def test_two_streams_share_one_plan(sw, gpu):
+    """Round trips on two streams at once on one plan (each stream has its own Fourier
+    scratch) equal the same round trips run one after the other, bit for bit."""
+    S = plan(sw, "T127", "fp16x3")
+    K = 32
+    ins = [to_dev(sw, sw.seeded_spectrum(S.spectrum, K, seed=40 + i), S.spectrum) for i in range(2)]
+    refs = [sw.transform(sw.transform(c, S), S).data.clone() for c in ins]
+    fields = [sw.Field(torch.empty((K, S.grid.npoints), device="cuda"), S.grid) for _ in range(2)]
+    outs = [sw.LowerTriangularArray.zeros(S.spectrum, K, device="cuda") for _ in range(2)]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    torch.cuda.synchronize()
+    for rep in range(4):
+        for i in (0, 1):
+            with torch.cuda.stream(streams[i]):
+                sw.transform_(fields[i], ins[i], S)
+                sw.transform_(outs[i], fields[i], S)
+    torch.cuda.synchronize()
+    for i in (0, 1):
+        assert torch.equal(outs[i].data, refs[i]), i
 
 
 def test_y00_constant_grid(sw, gpu):

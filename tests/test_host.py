@@ -100,3 +100,31 @@ def test_no_cpu_fallback_without_gpu(sw):
         pytest.skip("GPU present")
     with pytest.raises(sw.SHTError):
         sw.SpectralTransform(sw.Spectrum(33, 32), sw.OctahedralGaussianGrid(24), nlayers=2)
+
+
+def test_tune_cache_file_round_trip(sw, tmp_path):
+    """Autotuner choices persist as one JSON object per line: a loaded file saves back with
+    the same choices; a malformed line is reported while the valid ones still load."""
+    import ctypes as C
+    import json
+    L = sw.lib()
+    keys = ["grid", "nlat_half", "lmax", "mmax", "nlayers", "precision", "K", "mfma_min", "eights", "odd_first",
+            "leg_xcd", "fft_pair", "fourier", "dft_cg"]
+    rows = [dict(zip(keys, [0, 192, 257, 256, 64, 1, 64, 17, 0, 1, 1, 1, 2, 8])),
+            dict(zip(keys, [1, 24, 33, 32, 8, 0, 8, 11, 1, 0, 0, 1, 0, 4]))]
+    src = tmp_path / "tune.jsonl"
+    src.write_text("".join(json.dumps(r) + "\n" for r in rows))
+    n = C.c_int32()
+    assert L.sht_tune_cache_load(str(src).encode(), C.byref(n)) == 0 and n.value == 2
+    out = tmp_path / "saved.jsonl"
+    assert L.sht_tune_cache_save(str(out).encode()) == 0
+    saved = [json.loads(x) for x in out.read_text().splitlines() if x.strip()]
+    for r in rows:
+        assert r in saved
+    bad = tmp_path / "bad.jsonl"
+    r3 = dict(rows[0], K=32)
+    bad.write_text(json.dumps(r3) + "\n" + '{"grid": 0, "K": 5}\n' + json.dumps(dict(rows[0], dft_cg=3)) + "\n")
+    assert L.sht_tune_cache_load(str(bad).encode(), C.byref(n)) == 1  # SHT_ERR_ARG
+    assert n.value == 1
+    assert b"malformed" in L.sht_last_error()
+    assert L.sht_tune_cache_load(b"/nonexistent/dir/x.jsonl", None) == 1
