@@ -115,6 +115,22 @@ int sht_plan_create_matrix(const sht_desc *desc, sht_plan **plan);
 int sht_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void *coeffs,
                  int64_t ld_spec, int32_t K, void *stream);
 
+/* Allocation-free transform (no reference counterpart): for callers that cannot allocate
+ * device memory -- code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  direction SHT_DIR_SYNTHESIS runs sht_synthesis, SHT_DIR_ANALYSIS runs
+ * sht_analysis (unscale_coslat must then be 0), same layouts, strides, streams and results;
+ * the call's Fourier scratch is a static device array of SHT_STATIC_SCRATCH_BYTES that each
+ * GPU holds from the library's load until the process ends (its address is looked up when
+ * the first plan on the device is created), so the call allocates nothing.  The array is
+ * shared by every sht_transform_static call on a GPU, whatever the plan or stream: calls
+ * that overlap on one GPU race, so serialise them (one stream, or events).  Nothing in it
+ * needs to start at zero (each stage writes every row the next one reads).  SHT_ERR_DIM if
+ * K columns need more than the array: plan rows sum(nlon/2+1) x K x 16 bytes (T255: K <= 108),
+ * or a dense plan's GEMM workspace. */
+#define SHT_STATIC_SCRATCH_BYTES (64ll << 20)
+int sht_transform_static(sht_plan *plan, int32_t direction, const void *in, int64_t ld_in, void *out,
+                         int64_t ld_out, int32_t K, int32_t unscale_coslat, void *stream);
+
 /* Pullbacks (adjoints) of the two transforms, replace spec2grid_pullback! and
  * grid2spec_pullback! (SpeedyTransforms/ext/SpeedyTransformsEnzymeExt.jl:263-319).
  * sht_spec2grid_pullback: coeffs_bar (+)= S^T field_bar for the synthesis S (with the same

@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libsht.so")
 SOURCES = [os.path.join(HERE, "csrc", f) for f in
-           ("sht_legendre.hip", "sht_fft.hip", "sht_dft.hip", "sht_spectral.hip", "sht_mlp.hip", "sht_plan.cpp")]
+           ("sht_legendre.hip", "sht_fft.hip", "sht_dft.hip", "sht_spectral.hip", "sht_mlp.hip", "sht_static.hip", "sht_plan.cpp")]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

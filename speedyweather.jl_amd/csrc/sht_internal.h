@@ -223,6 +223,9 @@ int transform_with_scratch(sht_plan *plan, int direction, const void *in, int64_
                            int32_t K, int32_t unscale_coslat, void *scratch, void *stream);
 size_t plan_scratch_bytes(const sht_plan *plan, int K);
 int plan_device(const sht_plan *plan);
+// Looks up the address of the static scratch array (sht_static.hip) on `device`, which
+// must be current; once per device, later calls return at once.
+int static_scratch_setup(int device);
 
 // Kernel entry points.  Every stage kernel takes
 //   (const DevPlan *plans, CallArgs single, const CallArgs *calls, const Tile *tiles)

@@ -1028,6 +1028,7 @@ static int create_plan(const sht_desc *desc, bool matrix, sht_plan **plan) {
     rc = build_plan(pl);
     if (!rc && matrix) rc = build_matrices(pl);
     if (!rc && !stream_scratch(pl, nullptr)) rc = SHT_ERR_ALLOC;
+    if (!rc) rc = sht::static_scratch_setup(desc->device);
     if (rc) { free_plan(pl); return rc; }
     *plan = pl;
     return SHT_OK;
