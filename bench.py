@@ -264,12 +264,14 @@ def bench_c5(sw, seeds=200, heldout=8, steps=3000):
     torch.cuda.synchronize()
     t_fit = time.perf_counter() - t0
     held = ln.ab_score(net, nx, ny, xv, yv)
-    fc = ln.forecast_score(net, nx, ny, list(range(20000, 20008)))
+    # how the correction couples into the model: chosen on validation runs, scored on others
+    variant, kw, val = ln.choose_corrector(net, nx, ny, list(range(30000, 30008)))
+    fc = ln.forecast_score(net, nx, ny, list(range(20000, 20008)), variants={variant: kw})[variant]
     # step cost of the coarse model with and without the correction (8 members)
     m = dy.DryPrimitiveModel(31, 8, nmembers=8)
     m.set_state(m.jet_state(list(range(8))))
     m.run(4)
-    corr = ln.corrector(net, nx, ny)
+    corr = ln.corrector(net, nx, ny, **kw)
     ms = {}
     for tag, c in (("plain", None), ("corrected", corr)):
         torch.cuda.synchronize()
@@ -283,6 +285,7 @@ def bench_c5(sw, seeds=200, heldout=8, steps=3000):
                         f"{seeds} T127 L8 runs x 2 samples ({x.shape[0]} columns), {steps} Adam steps of 4096",
             "data_gen_s": t_gen, "train_s": t_fit, "train_columns_per_s": steps * 4096 / t_fit,
             "final_loss": losses[-1], "heldout_mse_ratio": held, "forecast_1h_error_ratio": fc,
+            "coupling": variant, "coupling_validation_ratios": val,
             "step_ms": ms}
 
 
@@ -310,6 +313,96 @@ def profiled_ms(kernel, precision):
     """Mean GPU duration (ms) of `kernel` in the committed rocprofv3 kernel-trace summary."""
     r = profile_table("kernel_times_t255_k64.txt").get((kernel, precision))
     return float(r[0]) * 1e-3 if r else None
+
+
+MODE_NAMES = {"0": "fp32", "1": "fp16x3", "2": "bf16x3"}
+
+
+def summarize_trace(path):
+    """Per-(stage kernel, precision) launch statistics of a rocprofv3 kernel trace
+    (kernel_trace.csv) of a `--no-extra --tune-cache` run, over the most frequent launch
+    shape: lines `kernel precision mean_us median_us launches workgroups vgpr lds_bytes`
+    (the figures profiled_ms reads)."""
+    import csv
+    groups = {}
+    for kernel, mode, r in _stage_dispatches(list(csv.DictReader(open(path))), "Start_Timestamp"):
+        wg = int(r["Grid_Size_X"]) // max(int(r["Workgroup_Size_X"]), 1)
+        g = groups.setdefault((kernel, mode), {}).setdefault((wg, r["VGPR_Count"], r["LDS_Block_Size"]), [])
+        g.append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3)
+    lines = ["# kernel precision mean_us median_us launches workgroups vgpr lds_bytes"]
+    for (kernel, mode), byshape in sorted(groups.items()):
+        (wg, vgpr, lds), d = max(byshape.items(), key=lambda kv: len(kv[1]))
+        lines.append(f"{kernel} {mode} {np.mean(d):.2f} {np.median(d):.2f} {len(d)} {wg} {vgpr} {lds}")
+    return "\n".join(lines)
+
+
+def _stage_dispatches(rows, key_start):
+    """(kernel, precision, launch shape, row) of the stage-kernel launches in `rows`, in launch
+    order: the Legendre kernels carry the precision mode as their first template argument;
+    an FFT launch belongs to the mode of the Legendre launch before it."""
+    import re
+    pat = re.compile(r"sht::(\w+)_kernel<([^>]*)>")
+    mode = None
+    for r in sorted(rows, key=lambda r: int(r[key_start])):
+        mt = pat.search(r["Kernel_Name"])
+        if not mt:
+            continue
+        name, targs = mt.group(1), [a.strip() for a in mt.group(2).split(",")]
+        if name.startswith("legendre"):
+            mode = MODE_NAMES.get(targs[0], targs[0])
+        if mode is None:
+            continue
+        kernel = {"dft_synthesis": "fft_synthesis", "dft_analysis": "fft_analysis"}.get(name, name)
+        yield kernel, mode, r
+
+
+def summarize_counters(paths):
+    """Per-(stage kernel, precision) means per launch of rocprofv3 --pmc counter collections
+    (counter_collection.csv of `--no-extra --tune-cache --copy-calibration` runs), over the most
+    frequent launch shape, with the derived figures: HBM bytes (FETCH_SIZE and WRITE_SIZE in
+    KiB, each scaled by what it reads for the largest plain device copy of the run, whose bytes
+    are known), MFMA busy share (SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x 1024 SIMDs)),
+    MFMA flops (SQ_INSTS_VALU_MFMA_MOPS_* x 512) and the share of LDS-active cycles stalled by
+    bank conflicts (SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE)."""
+    import csv
+    per = {}   # dispatch -> {"row": row, counters...}
+    scale = {}
+    for path in paths:
+        disp = {}
+        for r in csv.DictReader(open(path)):
+            d = disp.setdefault(r["Dispatch_Id"], {"row": r})
+            d[r["Counter_Name"]] = float(r["Counter_Value"])
+        copies = [d for d in disp.values() if d["row"]["Kernel_Name"].startswith("__amd_rocclr_copyBuffer")]
+        if copies:  # the calibration copy: 256 MiB read and written (the largest copy of the run)
+            big = max(copies, key=lambda d: int(d["row"]["Grid_Size"]))
+            for c in ("FETCH_SIZE", "WRITE_SIZE"):
+                if c in big and big[c] > 0:
+                    scale[c] = (256 << 20) / (big[c] * 1024)
+        for k, d in disp.items():
+            per.setdefault(k, {"row": d["row"]}).update({c: v for c, v in d.items() if c != "row"})
+    rows = [dict(d["row"], _c=d) for d in per.values()]
+    groups = {}
+    for kernel, mode, r in _stage_dispatches(rows, "Dispatch_Id"):
+        shape = (int(r["Grid_Size"]) // max(int(r["Workgroup_Size"]), 1), r["VGPR_Count"], r["LDS_Block_Size"])
+        groups.setdefault((kernel, mode), {}).setdefault(shape, []).append(r["_c"])
+    lines = [f"# copy calibration: bytes = FETCH_SIZE x 1024 x {scale.get('FETCH_SIZE', float('nan')):.4f}, "
+             f"WRITE_SIZE x 1024 x {scale.get('WRITE_SIZE', float('nan')):.4f}",
+             "# kernel precision launches workgroups fetch_bytes write_bytes mfma_busy_pct mfma_gflop "
+             "lds_conflict_pct_of_active"]
+    for (kernel, mode), byshape in sorted(groups.items()):
+        (wg, vgpr, lds), ds = max(byshape.items(), key=lambda kv: len(kv[1]))
+
+        def mean(c):
+            v = [d[c] for d in ds if c in d]
+            return float(np.mean(v)) if v else float("nan")
+        fetch = mean("FETCH_SIZE") * 1024 * scale.get("FETCH_SIZE", float("nan"))
+        write = mean("WRITE_SIZE") * 1024 * scale.get("WRITE_SIZE", float("nan"))
+        busy = 100 * mean("SQ_VALU_MFMA_BUSY_CYCLES") / (mean("GRBM_GUI_ACTIVE") * 1024)
+        flop = 512 * sum(mean(c) for c in ("SQ_INSTS_VALU_MFMA_MOPS_F16", "SQ_INSTS_VALU_MFMA_MOPS_BF16",
+                                             "SQ_INSTS_VALU_MFMA_MOPS_F32")) * 1e-9
+        conf = 100 * mean("SQ_LDS_BANK_CONFLICT") / mean("SQ_LDS_IDX_ACTIVE")
+        lines.append(f"{kernel} {mode} {len(ds)} {wg} {fetch:.0f} {write:.0f} {busy:.1f} {flop:.3f} {conf:.1f}")
+    return "\n".join(lines)
 
 
 def cpu_model():
@@ -365,7 +458,20 @@ def main():
     ap.add_argument("--c5-seeds", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run three plain 256 MiB device copies (scale for HBM byte counters)")
+    ap.add_argument("--tune-cache", default=None,
+                    help="autotuner cache file: loaded first if it exists (plans then tune without timing), "
+                         "written at the end")
+    ap.add_argument("--summarize-trace", default=None, metavar="KERNEL_TRACE_CSV",
+                    help="print the per-kernel summary of a rocprofv3 kernel trace and exit")
+    ap.add_argument("--summarize-counters", nargs="+", default=None, metavar="COUNTER_COLLECTION_CSV",
+                    help="print the per-kernel summary of rocprofv3 --pmc collections and exit")
     args = ap.parse_args()
+    if args.summarize_trace:
+        print(summarize_trace(args.summarize_trace))
+        return
+    if args.summarize_counters:
+        print(summarize_counters(args.summarize_counters))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
               "the transform shards over columns without communication, see the design notes",
@@ -377,6 +483,8 @@ def main():
     from oracle.sht_oracle import Grid as OGrid, OracleTransform, Spectrum as OSpectrum
     sw = __graft_entry__.load_package()
     torch.cuda.set_device(0)
+    if args.tune_cache and os.path.exists(args.tune_cache):
+        sw.load_tuning_cache(args.tune_cache)
 
     lmax, mmax, n, layers, fields = 257, 256, 192, 16, 4
     K = layers * fields
@@ -488,6 +596,8 @@ def main():
         cb = cpu_baseline(OSpectrum(lmax, mmax), OGrid("octahedral", n), coeffs, args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = value / cb["value"]
+    if args.tune_cache:
+        sw.save_tuning_cache(args.tune_cache)
     print(json.dumps(out), flush=True)
 
 

@@ -48,11 +48,17 @@ extern "C" {
 #define SHT_GRID_OCTAHEDRAL_GAUSSIAN 0
 #define SHT_GRID_FULL_GAUSSIAN 1
 
-/* Legendre shortcuts (SpeedyTransforms/src/legendre_shortcuts.jl:21-40):
- * mmax_truncation(ring) = min(nlon / d, mmax - 1) with d = 2, 3, 4 */
+/* Legendre shortcuts (SpeedyTransforms/src/legendre_shortcuts.jl:21-49): the last order
+ * kept on a ring, mmax_truncation(ring) = min(s(nlon, latd), mmax - 1) with s = nlon ÷ 2,
+ * nlon ÷ 3, nlon ÷ 4, floor(nlon / (2 + cosd(latd)^2)), floor(nlon / (2 + 2 cosd(latd))).
+ * The last two need not be monotonic in the ring index (a ring nearer the equator may keep
+ * fewer orders than one nearer the pole); every ring still contributes exactly its own
+ * orders 0..mmax_truncation(ring). */
 #define SHT_SHORTCUT_LINEAR 0
 #define SHT_SHORTCUT_QUADRATIC 1
 #define SHT_SHORTCUT_CUBIC 2
+#define SHT_SHORTCUT_LINQUAD_COSLAT2 3
+#define SHT_SHORTCUT_LINCUB_COSLAT 4
 
 /* arithmetic of the Legendre contraction (FFT and all other steps are FP32) */
 #define SHT_PREC_FP32 0      /* FP32 products, FP32 accumulation (MFMA f32) */

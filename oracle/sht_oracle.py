@@ -19,7 +19,7 @@ It restates, loop structure for loop structure, the reference CPU path
   ``Legendre.λlm!`` of AssociatedLegendrePolynomials.jl v1.0.2 (not vendored): the
   spherical-harmonic normalised P_l^m with the Condon-Shortley phase, restated here by
   the textbook three-term recursion in FP64;
-* Legendre shortcut: ``SpeedyTransforms/src/legendre_shortcuts.jl:21-25`` (linear);
+* Legendre shortcuts: ``SpeedyTransforms/src/legendre_shortcuts.jl:21-49`` (all five);
 * inverse Legendre: ``SpeedyTransforms/src/legendre.jl:2-33,38-101``;
 * forward Legendre: ``SpeedyTransforms/src/legendre.jl:104-126,131-198``;
 * Fourier stage: ``SpeedyTransforms/src/fourier.jl:27-77,129-250`` with FFTW's
@@ -171,9 +171,32 @@ def legendre_lambda(lmax: int, mmax: int, x) -> np.ndarray:
     return out[:, 0] if np.ndim(x) == 0 else out
 
 
+SHORTCUTS = ("linear", "quadratic", "cubic", "linquadcoslat2", "lincubcoslat")
+
+
+def legendre_shortcut(name: str, nlon, latd) -> np.ndarray:
+    """Last order kept per ring (SpeedyTransforms/src/legendre_shortcuts.jl:21-49): nlon ÷ 2,
+    ÷ 3, ÷ 4, floor(nlon / (2 + cosd(latd)^2)), floor(nlon / (2 + 2 cosd(latd)))."""
+    nlon = np.asarray(nlon, dtype=np.int64)
+    c = np.cos(np.radians(np.asarray(latd, dtype=np.float64)))
+    if name == "linear":
+        return nlon // 2
+    if name == "quadratic":
+        return nlon // 3
+    if name == "cubic":
+        return nlon // 4
+    if name == "linquadcoslat2":
+        return np.floor(nlon / (2.0 + c * c)).astype(np.int64)
+    if name == "lincubcoslat":
+        return np.floor(nlon / (2.0 + 2.0 * c)).astype(np.int64)
+    raise ValueError(f"unknown Legendre shortcut {name}")
+
+
 class OracleTransform:
     """FP64 restatement of SpectralTransform + transform! (spectral_transform.jl:118-302,
-    515-631) for octahedral / full Gaussian grids with the linear Legendre shortcut."""
+    515-631) for octahedral / full Gaussian grids with any of the five Legendre shortcuts
+    (each ring keeps its own orders, also where the latitude-dependent ones are not
+    monotonic in the ring index)."""
 
     def __init__(self, spectrum: Spectrum, grid: Grid, shortcut: str = "linear"):
         self.spectrum, self.grid = spectrum, grid
@@ -187,8 +210,8 @@ class OracleTransform:
         self.coslat_inv = 1.0 / self.coslat
         self.weights = w
         self.solid_angles = w * 2 * np.pi / grid.nlons  # quadrature_weights.jl:48-49
-        div = {"linear": 2, "quadratic": 3, "cubic": 4}[shortcut]
-        self.mtrunc = np.minimum(grid.nlons_half // div, spectrum.mmax - 1)  # :154-155
+        self.mtrunc = np.minimum(legendre_shortcut(shortcut, grid.nlons_half, latd[:n]),
+                                 spectrum.mmax - 1)  # spectral_transform.jl:154-155
         self.nfreq = grid.nlons_half // 2 + 1
         # Λ.data[lm, j] for the northern rings (spectral_transform.jl:167-172)
         self.legendre = legendre_lambda(spectrum.lmax, spectrum.mmax, self.cos_colat[:n])

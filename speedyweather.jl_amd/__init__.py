@@ -8,8 +8,8 @@ from .mlp import ColumnMLP
 from .spectrum import LowerTriangularArray, Spectrum
 from .synthetic import seeded_grid, seeded_spectrum
 from .transform import (MatrixSpectralTransform, SpectralBatch, SpectralTransform, UV_from_vordiv_, curl_, divergence_,
-                        gradient_, grid2spec_pullback_, inverse_laplace_, laplace_, spec2grid_pullback_, transform,
-                        transform_, transform_laplace_)
+                        gradient_, grid2spec_pullback_, inverse_laplace_, laplace_, load_tuning_cache,
+                        save_tuning_cache, spec2grid_pullback_, transform, transform_, transform_laplace_)
 
 __all__ = [
     "ColumnMLP", "MatrixSpectralTransform", "DimensionMismatch", "SHTError", "lib", "Field", "FullGaussianGrid", "OctahedralGaussianGrid",
@@ -17,5 +17,5 @@ __all__ = [
     "seeded_grid", "seeded_spectrum", "SpectralBatch", "SpectralTransform", "UV_from_vordiv_", "transform",
     "spec2grid_pullback_", "grid2spec_pullback_", "divergence_", "curl_", "laplace_", "inverse_laplace_",
     "gradient_",
-    "transform_", "transform_laplace_",
+    "transform_", "transform_laplace_", "save_tuning_cache", "load_tuning_cache",
 ]

@@ -16,7 +16,8 @@ SHT_ERR_UNSUPPORTED = 5
 SHT_ERR_ALLOC = 6
 
 GRIDS = {"OctahedralGaussianGrid": 0, "FullGaussianGrid": 1}
-SHORTCUTS = {"linear": 0, "quadratic": 1, "cubic": 2}
+# LegendreShortcut by short_name (legendre_shortcuts.jl:21-49); "linquadcoslat2" spells the ²
+SHORTCUTS = {"linear": 0, "quadratic": 1, "cubic": 2, "linquadcoslat2": 3, "linquadcoslat²": 3, "lincubcoslat": 4}
 PRECISIONS = {"fp32": 0, "fp16x3": 1, "bf16x3": 2}
 
 # every symbol include/sht.h declares

@@ -10,6 +10,7 @@ LIB = os.path.join(HERE, "libsht.so")
 SOURCES = [os.path.join(HERE, "csrc", f) for f in
            ("sht_legendre.hip", "sht_fft.hip", "sht_dft.hip", "sht_spectral.hip", "sht_mlp.hip", "sht_static.hip", "sht_plan.cpp")]
 
+ROCM_LIB = "/opt/rocm/lib"  # roctx ranges (rocprofiler-sdk-roctx)
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions stay disabled in device code
@@ -21,7 +22,7 @@ HIPCC_FLAGS = [
 def build_library(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     cmd = [hipcc, *HIPCC_FLAGS, "-I", os.path.join(REPO, "include"), "-x", "hip", *SOURCES,
-           "-o", LIB + ".tmp"]
+           "-L", ROCM_LIB, "-lrocprofiler-sdk-roctx", "-Wl,-rpath," + ROCM_LIB, "-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

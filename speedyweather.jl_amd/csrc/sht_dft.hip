@@ -398,9 +398,15 @@ __global__ __launch_bounds__(256) void dft_analysis_kernel(const DevPlan *__rest
         SHT_GLOBAL float4 *fb = as_global(static_cast<float4 *>(C.fbuf)) + as_global(P.fboff)[G.j] * (long long)K + G.k0;
         // adjoint of the synthesis (spec2grid pullback): rows times s_m (1/cos lat) / ΔΩ_j
         const float wj = C.adj ? (C.unscale ? as_global(P.coslat_inv)[G.j] : 1.f) / as_global(P.domega)[G.j] : 1.f;
-        for (int i = tid; i < CG * (mt + 1); i += 256) {
+        // rows mt < m <= mte: zeros (a non-monotonic shortcut's orders of rings nearer the pole)
+        const int mte = as_global(P.mte)[G.j];
+        for (int i = tid; i < CG * (mte + 1); i += 256) {
             const int kc = i % CG, mm = i / CG;
             if (kc >= ncol) continue;
+            if (mm > mt) {
+                gstore(fb + (unsigned)(mm * K + kc), make_float4(0.f, 0.f, 0.f, 0.f));
+                continue;
+            }
             const float2 pv = Fs[kc * FS + mm];
             const float2 qv = mm == 0 ? pv : Fs[kc * FS + mt + mm];
             // f_N = (Z_m + conj Z_{N-m})/2, f_S = (Z_m - conj Z_{N-m})/(2i)

@@ -562,6 +562,11 @@ struct FFTTile {
     int j, N, mt, k0, ncol;
 };
 
+// Last Fourier row of ring j the analysis writes: rows mt_j < m <= mte_j (orders that
+// rings nearer the pole keep under a non-monotonic shortcut) are written as zeros, since
+// the Legendre analysis sums every ring from the first one of an order on.
+__device__ inline int fft_rows_written(const DevPlan &P, int j) { return as_global(P.mte)[j]; }
+
 // Single-call launch orders, rings heaviest first.  Paired (synthesis, which reads the
 // Fourier rows): units (ring rank r, column-group pair p) go round-robin to the XCDs and
 // the two column groups of a unit, which share the 128-byte lines of the Fourier rows, run
@@ -770,10 +775,15 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_analysis_kernel(const DevP
         const float2 *z = b0 + col * N;
         // adjoint of the synthesis (spec2grid pullback): rows times s_m (1/cos lat) / ΔΩ_j
         const float wj = C.adj ? (C.unscale ? as_global(P.coslat_inv)[G.j] : 1.f) / as_global(P.domega)[G.j] : 1.f;
+        const int mte = fft_rows_written(P, G.j);
 #pragma unroll
         for (int u = 0; u < kFFTRowPasses; ++u) {
             const int mm = m0 + kFFTRowsPerPass * u;
-            if (col >= ncol || mm > mt) continue;
+            if (col >= ncol || mm > mte) continue;
+            if (mm > mt) {
+                gstore(fb + (unsigned)(mm * K + col), make_float4(0.f, 0.f, 0.f, 0.f));
+                continue;
+            }
             const float2 p = z[mm];
             const float2 q = z[mm == 0 ? 0 : N - mm];
             // f_N = (Z_m + conj Z_{N-m})/2, f_S = (Z_m - conj Z_{N-m})/(2i)

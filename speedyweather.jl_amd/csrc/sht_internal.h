@@ -5,9 +5,20 @@
 
 #include <string>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "../../include/sht.h"
 
 namespace sht {
+
+// roctx range over one host-side entry point (visible with rocprofv3 --marker-trace;
+// negligible without a tool attached)
+struct TraceRange {
+    explicit TraceRange(const char *name) { roctxRangePush(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange &) = delete;
+    TraceRange &operator=(const TraceRange &) = delete;
+};
 
 // Column group: 32 complex columns = 64 real MFMA columns per Legendre workgroup.
 constexpr int kColsComplex = 32;
@@ -157,6 +168,8 @@ struct DevPlan {
     // per northern ring j
     const int *nlon;
     const int *mt;            // 0-based last order kept (Legendre shortcut)
+    const int *mte;           // max of mt over rings 0..j: the FFT analysis writes rows
+                              // mt < m <= mte as zeros (Legendre stage rings of order m)
     const long long *fboff;   // first Fourier-buffer row of ring j (sum of nfreq before j)
     const long long *ring_n;  // first grid point of ring j
     const long long *ring_s;  // first grid point of the mirrored southern ring

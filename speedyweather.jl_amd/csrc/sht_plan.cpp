@@ -46,9 +46,14 @@ namespace {
 struct Geometry {
     int n = 0, nlat = 0, lmax = 0, mmax = 0, nharm = 0, nlon_max = 0;
     long long npoints = 0, fb_rows = 0;
-    std::vector<int> nlon, mt, j0;
+    // mt: last order kept on ring j (the Legendre shortcut); mte: its running maximum over
+    // rings 0..j, which decides the rings of an order in the Legendre stage (j0)
+    std::vector<int> nlon, mt, mte, j0;
     std::vector<long long> ring_n, ring_s, fboff, lm_off;
+    bool monotonic = true;
 };
+
+void gauss_legendre(int N, std::vector<double> &x, std::vector<double> &w);
 
 int make_geometry(const sht_desc *d, Geometry &g) {
     if (!d) return fail(SHT_ERR_ARG, "null descriptor");
@@ -58,7 +63,7 @@ int make_geometry(const sht_desc *d, Geometry &g) {
     if (d->mmax < 1 || d->lmax < d->mmax || d->lmax > 8192)
         return fail(SHT_ERR_ARG, "need 1 <= mmax <= lmax <= 8192");
     if (d->nlayers < 1) return fail(SHT_ERR_ARG, "nlayers must be >= 1");
-    if (d->shortcut < SHT_SHORTCUT_LINEAR || d->shortcut > SHT_SHORTCUT_CUBIC)
+    if (d->shortcut < SHT_SHORTCUT_LINEAR || d->shortcut > SHT_SHORTCUT_LINCUB_COSLAT)
         return fail(SHT_ERR_UNSUPPORTED, "unknown Legendre shortcut");
     if (d->precision < SHT_PREC_FP32 || d->precision > SHT_PREC_BF16X3)
         return fail(SHT_ERR_ARG, "unknown precision mode");
@@ -79,12 +84,34 @@ int make_geometry(const sht_desc *d, Geometry &g) {
     for (int j = 0; j < n; ++j) { g.ring_n[j] = pos; pos += g.nlon[j]; }
     for (int j = n - 1; j >= 0; --j) { g.ring_s[j] = pos; pos += g.nlon[j]; }
     g.npoints = pos;
-    // Legendre shortcut: mt_j = min(nlon_j / div, mmax - 1)
-    const int div = d->shortcut == SHT_SHORTCUT_LINEAR ? 2 : (d->shortcut == SHT_SHORTCUT_QUADRATIC ? 3 : 4);
+    // Legendre shortcut (legendre_shortcuts.jl:21-49; spectral_transform.jl:154-155):
+    // mt_j = min(shortcut(nlon_j, latd_j), mmax - 1), with latd_j = acosd(x_j) - 90 of the
+    // ascending Gauss nodes x_j (full_gaussian.jl:48-50) for the latitude-dependent ones
     g.mt.resize(n);
-    for (int j = 0; j < n; ++j) g.mt[j] = std::min(g.nlon[j] / div, g.mmax - 1);
-    for (int j = 1; j < n; ++j)
-        if (g.mt[j] < g.mt[j - 1]) return fail(SHT_ERR_UNSUPPORTED, "non-monotonic shortcut");
+    std::vector<double> xg, wg;
+    if (d->shortcut >= SHT_SHORTCUT_LINQUAD_COSLAT2) gauss_legendre(2 * n, xg, wg);
+    for (int j = 0; j < n; ++j) {
+        int s;
+        switch (d->shortcut) {
+            case SHT_SHORTCUT_LINEAR: s = g.nlon[j] / 2; break;
+            case SHT_SHORTCUT_QUADRATIC: s = g.nlon[j] / 3; break;
+            case SHT_SHORTCUT_CUBIC: s = g.nlon[j] / 4; break;
+            default: {
+                const double latd = std::acos(xg[j]) * (180.0 / M_PI) - 90.0;  // ascending nodes: north first
+                const double c = std::cos(latd * (M_PI / 180.0));
+                const double den = d->shortcut == SHT_SHORTCUT_LINQUAD_COSLAT2 ? 2.0 + c * c : 2.0 + 2.0 * c;
+                s = (int)std::floor((double)g.nlon[j] / den);
+            }
+        }
+        g.mt[j] = std::min(s, g.mmax - 1);
+    }
+    g.mte.resize(n);
+    for (int j = 0; j < n; ++j) {
+        g.mte[j] = j == 0 ? g.mt[0] : std::max(g.mte[j - 1], g.mt[j]);
+        if (g.mte[j] != g.mt[j]) g.monotonic = false;
+        // the Legendre stage writes orders up to mte_j into ring j's Fourier rows
+        if (g.mte[j] > g.nlon[j] / 2) return fail(SHT_ERR_UNSUPPORTED, "shortcut keeps more orders than a ring holds");
+    }
     g.fboff.resize(n);
     long long rows = 0;
     for (int j = 0; j < n; ++j) { g.fboff[j] = rows; rows += g.nlon[j] / 2 + 1; }
@@ -92,7 +119,7 @@ int make_geometry(const sht_desc *d, Geometry &g) {
     g.j0.resize(g.mmax);
     for (int m = 0; m < g.mmax; ++m) {
         int j = 0;
-        while (j < n && g.mt[j] < m) ++j;
+        while (j < n && g.mte[j] < m) ++j;
         g.j0[m] = j;
     }
     g.lm_off.resize(g.mmax);
@@ -775,6 +802,7 @@ int build_plan(sht_plan *pl) {
     P.n = n; P.lmax = lmax; P.mmax = mmax; P.nharm = g.nharm; P.npoints = g.npoints;
     if ((rc = dev_upload(A, g.nlon, &P.nlon))) return rc;
     if ((rc = dev_upload(A, g.mt, &P.mt))) return rc;
+    if ((rc = dev_upload(A, g.mte, &P.mte))) return rc;
     if ((rc = dev_upload(A, g.fboff, &P.fboff))) return rc;
     if ((rc = dev_upload(A, g.ring_n, &P.ring_n))) return rc;
     if ((rc = dev_upload(A, g.ring_s, &P.ring_s))) return rc;
@@ -1008,6 +1036,7 @@ static int build_matrices(sht_plan *pl) {
 
 // A regular plan, or with matrix the dense one, with the null stream's scratch allocated.
 static int create_plan(const sht_desc *desc, bool matrix, sht_plan **plan) {
+    sht::TraceRange trace_range(matrix ? "sht_plan_create_matrix" : "sht_plan_create");
     if (!plan) return fail(SHT_ERR_ARG, "null plan pointer");
     *plan = nullptr;
     sht_plan *pl = new (std::nothrow) sht_plan();
@@ -1039,6 +1068,7 @@ int sht_plan_create(const sht_desc *desc, sht_plan **plan) { return create_plan(
 int sht_plan_create_matrix(const sht_desc *desc, sht_plan **plan) { return create_plan(desc, true, plan); }
 
 int sht_plan_destroy(sht_plan *plan) {
+    sht::TraceRange trace_range("sht_plan_destroy");
     if (!plan) return SHT_OK;
     DeviceGuard guard(plan->d.device);
     (void)hipDeviceSynchronize();
@@ -1184,21 +1214,25 @@ static int run_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void
 
 int sht_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *field, int64_t ld_grid, int32_t K,
                   int32_t unscale_coslat, void *stream) {
+    sht::TraceRange trace_range("sht_synthesis");
     return run_synthesis(plan, coeffs, ld_spec, field, ld_grid, K, unscale_coslat, 0, 0, stream);
 }
 
 int sht_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void *coeffs, int64_t ld_spec, int32_t K,
                  void *stream) {
+    sht::TraceRange trace_range("sht_analysis");
     return run_analysis(plan, field, ld_grid, coeffs, ld_spec, K, 0, 0, 0, stream);
 }
 
 int sht_spec2grid_pullback(sht_plan *plan, const void *field_bar, int64_t ld_grid, void *coeffs_bar, int64_t ld_spec,
                            int32_t K, int32_t unscale_coslat, int32_t accumulate, void *stream) {
+    sht::TraceRange trace_range("sht_spec2grid_pullback");
     return run_analysis(plan, field_bar, ld_grid, coeffs_bar, ld_spec, K, unscale_coslat, 1, accumulate, stream);
 }
 
 int sht_grid2spec_pullback(sht_plan *plan, const void *coeffs_bar, int64_t ld_spec, void *field_bar, int64_t ld_grid,
                            int32_t K, int32_t accumulate, void *stream) {
+    sht::TraceRange trace_range("sht_grid2spec_pullback");
     return run_synthesis(plan, coeffs_bar, ld_spec, field_bar, ld_grid, K, 0, 2, accumulate, stream);
 }
 
@@ -1236,6 +1270,7 @@ static int lap_op(int32_t inverse, int32_t flipsign, double radius, LapOp &op) {
 
 int sht_synthesis_laplace(sht_plan *plan, const void *coeffs, int64_t ld_spec, void *field, int64_t ld_grid, int32_t K,
                           int32_t inverse, int32_t flipsign, double radius, int32_t unscale_coslat, void *stream) {
+    sht::TraceRange trace_range("sht_synthesis_laplace");
     LapOp op;
     const int rc = lap_op(inverse, flipsign, radius, op);
     if (rc) return rc;
@@ -1244,6 +1279,7 @@ int sht_synthesis_laplace(sht_plan *plan, const void *coeffs, int64_t ld_spec, v
 
 int sht_analysis_laplace(sht_plan *plan, const void *field, int64_t ld_grid, void *coeffs, int64_t ld_spec, int32_t K,
                          int32_t inverse, int32_t flipsign, int32_t add, double radius, void *stream) {
+    sht::TraceRange trace_range("sht_analysis_laplace");
     LapOp op;
     const int rc = lap_op(inverse, flipsign, radius, op);
     if (rc) return rc;
@@ -1252,6 +1288,7 @@ int sht_analysis_laplace(sht_plan *plan, const void *field, int64_t ld_grid, voi
 
 int sht_uv_from_vordiv(sht_plan *plan, const void *vor, const void *div, void *U, void *V, int64_t ld, int32_t K,
                        double radius, void *stream) {
+    sht::TraceRange trace_range("sht_uv_from_vordiv");
     int rc = check_call(plan, vor, div, K);
     if (rc) return rc;
     if (!U || !V) return fail(SHT_ERR_ARG, "null data pointer");
@@ -1264,6 +1301,7 @@ int sht_uv_from_vordiv(sht_plan *plan, const void *vor, const void *div, void *U
 
 int sht_divergence(sht_plan *plan, const void *u, const void *v, void *out, int64_t ld, int32_t K, int32_t curl,
                    int32_t flipsign, int32_t add, double radius, void *stream) {
+    sht::TraceRange trace_range("sht_divergence");
     int rc = check_call(plan, u, v, K);
     if (rc) return rc;
     if (!out) return fail(SHT_ERR_ARG, "null data pointer");
@@ -1280,6 +1318,7 @@ int sht_divergence(sht_plan *plan, const void *u, const void *v, void *out, int6
 
 int sht_laplace(sht_plan *plan, const void *alms, void *out, int64_t ld, int32_t K, int32_t inverse, int32_t flipsign,
                 int32_t add, double radius, void *stream) {
+    sht::TraceRange trace_range("sht_laplace");
     int rc = check_call(plan, alms, out, K);
     if (rc) return rc;
     if (ld < plan->g.nharm) return fail(SHT_ERR_DIM, "DimensionMismatch: ld < nharm");
@@ -1295,6 +1334,7 @@ int sht_laplace(sht_plan *plan, const void *alms, void *out, int64_t ld, int32_t
 
 int sht_gradient(sht_plan *plan, const void *p, void *dpdx, void *dpdy, int64_t ld, int32_t K, double radius,
                  void *stream) {
+    sht::TraceRange trace_range("sht_gradient");
     int rc = check_call(plan, p, dpdx, K);
     if (rc) return rc;
     if (!dpdy) return fail(SHT_ERR_ARG, "null data pointer");
@@ -1503,6 +1543,7 @@ int sht_tune_cache_load(const char *path, int32_t *nloaded) {
 }
 
 int sht_plan_autotune(sht_plan *plan, int32_t K, void *stream) {
+    sht::TraceRange trace_range("sht_plan_autotune");
     if (!plan) return fail(SHT_ERR_ARG, "null plan");
     if (K < 1 || K > plan->d.nlayers) return fail(SHT_ERR_DIM, "DimensionMismatch: K outside 1..nlayers");
     DeviceGuard guard(plan->d.device);
@@ -1797,6 +1838,7 @@ int build_batch(sht_batch *b, const sht_batch_item *items, int n, int flags) {
 extern "C" {
 
 int sht_batch_create(const sht_batch_item *items, int32_t n, int32_t flags, sht_batch **batch) {
+    sht::TraceRange trace_range("sht_batch_create");
     if (!batch) return fail(SHT_ERR_ARG, "null batch pointer");
     *batch = nullptr;
     if (!items || n < 1) return fail(SHT_ERR_ARG, "empty batch");
@@ -1831,6 +1873,7 @@ int sht_batch_create(const sht_batch_item *items, int32_t n, int32_t flags, sht_
 }
 
 int sht_batch_launch(sht_batch *b, void *stream) {
+    sht::TraceRange trace_range("sht_batch_launch");
     if (!b) return fail(SHT_ERR_ARG, "null batch");
     DeviceGuard guard(b->device);
     hipError_t e;

@@ -42,6 +42,7 @@ int sht::static_scratch_setup(int device) {
 
 extern "C" int sht_transform_static(sht_plan *plan, int32_t direction, const void *in, int64_t ld_in, void *out,
                                     int64_t ld_out, int32_t K, int32_t unscale_coslat, void *stream) {
+    sht::TraceRange trace_range("sht_transform_static");
     if (!plan) {
         sht::set_last_error("null plan");
         return SHT_ERR_ARG;

@@ -56,7 +56,9 @@ function shortcut_code(S)
     s === SpeedyTransforms.LegendreShortcutLinear && return Int32(0)
     s === SpeedyTransforms.LegendreShortcutQuadratic && return Int32(1)
     s === SpeedyTransforms.LegendreShortcutCubic && return Int32(2)
-    throw(ArgumentError("libsht implements the linear, quadratic and cubic Legendre shortcuts"))
+    s === SpeedyTransforms.LegendreShortcutLinQuadCoslat² && return Int32(3)
+    s === SpeedyTransforms.LegendreShortcutLinCubCoslat && return Int32(4)
+    throw(ArgumentError("libsht implements the five Legendre shortcuts of legendre_shortcuts.jl"))
 end
 
 function precision_code()

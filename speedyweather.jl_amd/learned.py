@@ -96,11 +96,13 @@ def generate(seeds, fine_trunc=127, coarse_trunc=31, spinup_hours=48.0, sample_e
     return torch.cat(xs), torch.cat(ys)
 
 
-def forecast_score(net, nx, ny, seeds, spinup_hours=48.0, window_hours=1.0, members=8, device="cuda:0"):
+def forecast_score(net, nx, ny, seeds, spinup_hours=48.0, window_hours=1.0, members=8, device="cuda:0",
+                   variants=None):
     """Error of the coarse model's forecast over the window from trunc(X(t)) against
     trunc(X_fine(t + W)), with the correction relative to without it (mean squared
     spectral coefficient error of vor, div, temp and ln ps, each normalised by the plain
-    run's error)."""
+    run's error).  variants: {name: corrector keyword arguments} scored from the same
+    states (returns {name: ratios}); default one corrector with its default arguments."""
     fine = DryPrimitiveModel(127, NL, nmembers=members, device=device)
     coarse = DryPrimitiveModel(31, NL, nmembers=members, device=device)
     W = window_hours * 3600.0
@@ -113,13 +115,41 @@ def forecast_score(net, nx, ny, seeds, spinup_hours=48.0, window_hours=1.0, memb
     fine.run(int(round(W / fine.dt)))
     ref = _truncated(fine, coarse)
     err = {}
-    for tag, corr in (("plain", None), ("corrected", corrector(net, nx, ny))):
+    runs = {"plain": None}
+    for name, kw in (variants or {"corrected": {}}).items():
+        runs[name] = corrector(net, nx, ny, **kw)
+    for tag, corr in runs.items():
         coarse.set_state(st0)
         coarse.run(int(round(W / coarse.dt)), correction=corr)
         err[tag] = {k: float((coarse.state[k] - ref[k]).abs().pow(2).mean()) for k in ref}
     fine.close()
     coarse.close()
-    return {k: err["corrected"][k] / err["plain"][k] for k in err["plain"]}
+    ratios = {t: {k: err[t][k] / err["plain"][k] for k in err["plain"]} for t in runs if t != "plain"}
+    return ratios if variants else ratios["corrected"]
+
+
+# How the correction couples into the model (corrector keyword arguments): all of its
+# divergence, or only the divergence's departure from the column mean (the column mean
+# drives the surface-pressure tendency through continuity), with or without its own ln ps
+# tendency.
+CORRECTOR_VARIANTS = {
+    "uvT": {"fields": ("vor", "div", "temp")},
+    "uvT_baroclinic_div": {"fields": ("vor", "div", "temp"), "baroclinic_div": True},
+    "uvTq_baroclinic_div": {"fields": ("vor", "div", "temp", "lnps"), "baroclinic_div": True},
+}
+
+
+def choose_corrector(net, nx, ny, seeds, **kw):
+    """Scores every CORRECTOR_VARIANTS entry on the forecast from `seeds` (validation runs,
+    neither training nor held-out) and returns (name, kwargs, scores): the lowest mean
+    ratio among the variants whose ln ps ratio is at most 1, else the lowest ln ps ratio."""
+    scores = forecast_score(net, nx, ny, seeds, variants=CORRECTOR_VARIANTS, **kw)
+    ok = [n for n, r in scores.items() if r["lnps"] <= 1.0]
+    if ok:
+        name = min(ok, key=lambda n: float(np.mean(list(scores[n].values()))))
+    else:
+        name = min(scores, key=lambda n: scores[n]["lnps"])
+    return name, CORRECTOR_VARIANTS[name], scores
 
 
 class Normaliser:
@@ -185,11 +215,12 @@ def ab_score(net, nx, ny, x, y):
     return out
 
 
-def corrector(net, nx, ny, fields=("vor", "div", "temp")):
+def corrector(net, nx, ny, fields=("vor", "div", "temp"), baroclinic_div=False):
     """A `correction` callback for DryPrimitiveModel.step: adds the MLP's column tendencies
     (u, v -> vor, div via curl / divergence of (u, v) / cos(lat); T; ln ps) to the
-    spectral tendencies of `fields`.  ln ps is left to the plain tendency by default: its
-    learned part does not beat it on held-out runs."""
+    spectral tendencies of `fields`.  ln ps is left to the plain tendency by default.
+    baroclinic_div: the divergence correction minus its mean over the (equally spaced)
+    sigma levels, so that it does not change the surface-pressure tendency."""
     def apply(model, tend, grid):
         E, L, npts = model.E, model.L, model.grid.npoints
         x = _columns(_grid_state(grid))
@@ -205,7 +236,11 @@ def corrector(net, nx, ny, fields=("vor", "div", "temp")):
         if "vor" in fields:
             tend["vor"] += model.ops.curl(us, vs) / model.a
         if "div" in fields:
-            tend["div"] += model.ops.divergence(us, vs) / model.a
+            dv = model.ops.divergence(us, vs) / model.a
+            if baroclinic_div:
+                dv = dv.view(E, L, -1)
+                dv = (dv - dv.mean(dim=1, keepdim=True)).view(EL, -1)
+            tend["div"] += dv
         if "temp" in fields:
             tend["temp"] += s[2 * EL:3 * EL] * keep
         if "lnps" in fields:

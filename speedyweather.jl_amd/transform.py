@@ -132,6 +132,21 @@ class SpectralTransform:
         return False
 
 
+def save_tuning_cache(path: str) -> None:
+    """Writes every autotuner choice made or loaded in this process to `path` (one JSON
+    object per line; sht_tune_cache_save)."""
+    check(lib().sht_tune_cache_save(str(path).encode()), "sht_tune_cache_save")
+
+
+def load_tuning_cache(path: str) -> int:
+    """Merges the choices in `path` into the process's autotuner cache, so autotune() of the
+    same size class applies them without timing (sht_tune_cache_load); returns how many
+    loaded."""
+    n = C.c_int32()
+    check(lib().sht_tune_cache_load(str(path).encode(), C.byref(n)), "sht_tune_cache_load")
+    return n.value
+
+
 class MatrixSpectralTransform(SpectralTransform):
     """Dense matrix transform (MatrixSpectralTransform, SpeedyTransforms/src/
     matrix_transform.jl:56-260): spectral <-> grid as one FP32 GEMM each with precomputed

@@ -78,6 +78,19 @@ def test_truncation_tendency_learned(mods):
     # and the corrected coarse model's one-hour forecast is closer to the truncated fine run
     fc = ln.forecast_score(net, nx, ny, list(range(700, 708)))
     assert fc["vor"] < 1.0 and fc["div"] < 1.0 and fc["temp"] < 1.0, fc
+    # the coupling variants: chosen on validation runs; the baroclinic divergence correction
+    # has no column mean (it leaves the surface-pressure tendency to the plain model)
+    name, kw, val = ln.choose_corrector(net, nx, ny, list(range(800, 808)))
+    assert name in ln.CORRECTOR_VARIANTS and set(val) == set(ln.CORRECTOR_VARIANTS)
+    m = dy.DryPrimitiveModel(31, 8, nmembers=2)
+    m.set_state(m.jet_state([3, 4]))
+    tend, g = m.tendencies(m.state, with_grid=True)
+    t0 = {k: v.clone() for k, v in tend.items()}
+    ln.corrector(net, nx, ny, baroclinic_div=True)(m, tend, g)
+    dd = (tend["div"] - t0["div"]).view(2, 8, -1)
+    assert float(dd.mean(dim=1).abs().max()) <= 1e-6 * float(dd.abs().max())
+    assert float(dd.abs().max()) > 0 and torch.equal(tend["lnps"], t0["lnps"])
+    m.close()
     # a corrected coarse run stays bounded
     m = dy.DryPrimitiveModel(31, 8, nmembers=2)
     m.set_state(m.jet_state([7, 8]))

@@ -373,3 +373,72 @@ def test_gradient_operators_against_finite_differences(lmax, mmax):
     # a planted mistake (the curl's signs in the divergence) fails the check above
     bad = 1j * gradient_factors(sp)[4] * u - (-gradient_factors(sp)[0] * np.roll(v, 1, axis=1))
     assert np.abs(_eval_order(sp, bad, 1, phi) - _eval_order(sp, div, 1, phi)).max() > 1e-3
+
+
+@pytest.mark.parametrize("kind, n, lmax, mmax, shortcut", [
+    ("full", 24, 33, 32, "lincubcoslat"),       # not monotonic in the ring index
+    ("octahedral", 24, 33, 32, "linquadcoslat2"),
+    ("octahedral", 48, 65, 64, "lincubcoslat"),
+    ("full", 24, 33, 32, "quadratic"),
+])
+def test_shortcut_keeps_each_rings_own_orders(kind, n, lmax, mmax, shortcut):
+    """Every ring keeps exactly its orders 0..mt_j (legendre_shortcuts.jl:21-49,
+    spectral_transform.jl:154-155), pinned ring by ring against the linear-shortcut transform
+    (which keeps every order the ring holds): synthesis on ring j equals the linear synthesis
+    of the coefficients with orders above mt_j zeroed, and analysis equals the sum over rings
+    of the linear analysis of the field on that ring pair alone, orders above mt_j zeroed."""
+    sp, grid = Spectrum(lmax, mmax), Grid(kind, n)
+    o = OracleTransform(sp, grid, shortcut)
+    lin = OracleTransform(sp, grid, "linear")
+    mt = o.mtrunc
+    assert (mt <= lin.mtrunc).all()
+    if shortcut == "lincubcoslat" and kind == "full":
+        assert (np.diff(mt) < 0).any()  # the case the ring-set bookkeeping has to cope with
+    # between the linear shortcut (cos lat -> 0) and the quadratic / cubic one (cos lat -> 1)
+    lo = {"linquadcoslat2": 3, "lincubcoslat": 4, "quadratic": 3}[shortcut]
+    assert (mt >= np.minimum(grid.nlons_half // lo, mmax - 1)).all()
+    rng = np.random.default_rng(3)
+    K = 3
+    a = rng.standard_normal((K, sp.nharm)) + 1j * rng.standard_normal((K, sp.nharm))
+    a[:, sp.offsets[0]:sp.offsets[0] + lmax] = a[:, sp.offsets[0]:sp.offsets[0] + lmax].real
+    f = o.synthesis(a)
+    m_of = np.repeat(np.arange(mmax), [lmax - m for m in range(mmax)])
+    f_ref = np.zeros_like(f)
+    for j in range(n):
+        fj = lin.synthesis(np.where(m_of[None, :] <= mt[j], a, 0))
+        for r in (j, grid.nlat - 1 - j):
+            s0 = grid.ring_start[r]
+            f_ref[:, s0:s0 + grid.nlons[r]] = fj[:, s0:s0 + grid.nlons[r]]
+    assert np.allclose(f, f_ref, rtol=0, atol=1e-12 * np.abs(f_ref).max())
+    g = rng.standard_normal((K, grid.npoints))
+    b = o.analysis(g)
+    b_ref = np.zeros_like(b)
+    for j in range(n):
+        gj = np.zeros_like(g)
+        for r in (j, grid.nlat - 1 - j):
+            s0 = grid.ring_start[r]
+            gj[:, s0:s0 + grid.nlons[r]] = g[:, s0:s0 + grid.nlons[r]]
+        b_ref += np.where(m_of[None, :] <= mt[j], lin.analysis(gj), 0)
+    assert np.allclose(b, b_ref, rtol=0, atol=1e-12 * np.abs(b_ref).max())
+    # the C restatement keeps the same orders
+    c = CRefTransform(o)
+    assert np.allclose(c.synthesis(a, np.float64), f, rtol=0, atol=1e-11 * np.abs(f).max())
+    assert np.allclose(c.analysis(g, np.float64), b, rtol=0, atol=1e-11 * np.abs(b).max())
+
+
+def test_shortcut_planted_mistake_fails():
+    """The ring-by-ring check above notices a ring that keeps one order too many."""
+    sp, grid = Spectrum(33, 32), Grid("full", 24)
+    o = OracleTransform(sp, grid, "lincubcoslat")
+    lin = OracleTransform(sp, grid, "linear")
+    o.mtrunc = o.mtrunc.copy()
+    j = int(np.argmin(o.mtrunc))
+    o.mtrunc[j] += 1
+    a = np.zeros((1, sp.nharm), complex)
+    m = o.mtrunc[j]
+    a[0, sp.offsets[m]] = 1.0  # l = m: nonzero on every ring
+    f = o.synthesis(a)
+    s0 = grid.ring_start[j]
+    m_of = np.repeat(np.arange(32), [33 - mm for mm in range(32)])
+    fj = lin.synthesis(np.where(m_of[None, :] <= m - 1, a, 0))
+    assert not np.allclose(f[:, s0:s0 + grid.nlons[j]], fj[:, s0:s0 + grid.nlons[j]])
