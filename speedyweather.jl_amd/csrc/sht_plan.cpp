@@ -280,7 +280,18 @@ struct LaunchUnit {
     std::vector<Tile> tiles;
 };
 
-static std::vector<Tile> xcd_order(std::vector<LaunchUnit> units) {
+// Light-first rotation of a heaviest-first list for `slots` resident workgroups: the tiles
+// beyond the slots (the lightest) move to the front, so that they finish early and the
+// tiles that start late are light ones starting early rather than tiles that wait for the
+// first heavy ones to end (which left a tail of few resident workgroups).  slots = 0: as is.
+template <class T>
+static void light_first(std::vector<T> &list, long long slots) {
+    if (slots <= 0 || (long long)list.size() <= slots) return;
+    const long long e = (long long)list.size() - slots;
+    std::rotate(list.begin(), list.end() - e, list.end());
+}
+
+static std::vector<Tile> xcd_order(std::vector<LaunchUnit> units, long long slots_per_xcd = 0) {
     std::stable_sort(units.begin(), units.end(),
                      [](const LaunchUnit &a, const LaunchUnit &b) { return a.cost > b.cost; });
     std::vector<std::vector<Tile>> lists(sht::kXCDs);
@@ -290,6 +301,7 @@ static std::vector<Tile> xcd_order(std::vector<LaunchUnit> units) {
         load[x] += u.cost;
         lists[x].insert(lists[x].end(), u.tiles.begin(), u.tiles.end());
     }
+    for (auto &l : lists) light_first(l, slots_per_xcd);
     size_t len = 0;
     for (const auto &l : lists) len = std::max(len, l.size());
     std::vector<Tile> out(len * sht::kXCDs, Tile{0, -1, 0, 0});
@@ -331,8 +343,10 @@ struct sht_plan {
     std::vector<void *> allocs;
     // single-call Legendre launch orders, one per column-group count cgs = 1..max, grouped
     // by order m per XCD ([1]) or plain heavy-first ([0])
-    std::vector<const Tile *> syn_tiles[2], ana_tiles[2];
-    std::vector<int> nsyn[2], nana[2];
+    // Legendre launch orders [variant][column groups - 1]: variant bit 0 groups the tiles of
+    // an order on one XCD, bit 1 rotates the lightest tiles beyond the resident slots first
+    std::vector<const Tile *> syn_tiles[4], ana_tiles[4];
+    std::vector<int> nsyn[4], nana[4];
     // tuned choices (sht_plan_autotune): FFT factorisation, Legendre order grouping, FFT
     // synthesis column-group pairing, Fourier kernel kind and its columns per workgroup
     int leg_xcd = 1, fft_pair = 1;
@@ -874,15 +888,29 @@ int build_plan(sht_plan *pl) {
     if ((rc = dev_upload(A, gyv, &P.gy))) return rc;
     if ((rc = dev_upload(A, gxv, &P.gx))) return rc;
     {
+        // resident workgroups of each Legendre kernel on the device (for the light-first orders)
+        long long slots_syn = 0, slots_ana = 0;
+        {
+            int cus = 0, bs = 0, ba = 0;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d.device) == hipSuccess &&
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &bs, sht::legendre_kernel(sht::kLegendreSynthesis, mode, false), 256, 0) == hipSuccess &&
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &ba, sht::legendre_kernel(sht::kLegendreAnalysis, mode, false), 256, 0) == hipSuccess) {
+                slots_syn = (long long)cus * bs;
+                slots_ana = (long long)cus * ba;
+            }
+        }
         const int cgs_max = (d.nlayers + sht::kColsComplex - 1) / sht::kColsComplex;
         for (int cgs = 1; cgs <= cgs_max; ++cgs)
-            for (int grouped = 0; grouped < 2; ++grouped) {
+            for (int variant = 0; variant < 4; ++variant) {
+                const bool grouped = (variant & 1) != 0, light = (variant & 2) != 0;
                 std::vector<LaunchUnit> su = legendre_units(syn_t, syn_c, 0, cgs);
                 std::vector<LaunchUnit> au = legendre_units(ana_t, ana_c, 0, cgs);
                 std::vector<Tile> so, ao;
                 if (grouped) {
-                    so = xcd_order(su);
-                    ao = xcd_order(au);
+                    so = xcd_order(su, light ? slots_syn / sht::kXCDs : 0);
+                    ao = xcd_order(au, light ? slots_ana / sht::kXCDs : 0);
                 } else {
                     // tiles heaviest first, one after another
                     for (auto *uo : {&su, &au}) {
@@ -893,14 +921,18 @@ int build_plan(sht_plan *pl) {
                         std::vector<Tile> &o = uo == &su ? so : ao;
                         for (auto &x : t) o.push_back(x.second);
                     }
+                    if (light) {
+                        light_first(so, slots_syn);
+                        light_first(ao, slots_ana);
+                    }
                 }
                 const Tile *ds = nullptr, *da = nullptr;
                 if ((rc = dev_upload(A, so, &ds))) return rc;
                 if ((rc = dev_upload(A, ao, &da))) return rc;
-                pl->syn_tiles[grouped].push_back(ds);
-                pl->ana_tiles[grouped].push_back(da);
-                pl->nsyn[grouped].push_back((int)so.size());
-                pl->nana[grouped].push_back((int)ao.size());
+                pl->syn_tiles[variant].push_back(ds);
+                pl->ana_tiles[variant].push_back(da);
+                pl->nsyn[variant].push_back((int)so.size());
+                pl->nana[variant].push_back((int)ao.size());
             }
     }
     {
@@ -1497,7 +1529,7 @@ bool tune_from_line(const std::string &line, TuneKey &k, TuneChoice &c) {
     c.fft_pair = (int)v[11];
     c.fourier = (int)v[12];
     c.dft_cg = (int)v[13];
-    return (c.pol.mfma_min == 11 || c.pol.mfma_min == 17) && c.leg_xcd >= 0 && c.leg_xcd <= 1 && c.fft_pair >= 0 &&
+    return (c.pol.mfma_min == 11 || c.pol.mfma_min == 17) && c.leg_xcd >= 0 && c.leg_xcd <= 3 && c.fft_pair >= 0 &&
            c.fft_pair <= 1 && c.fourier >= 0 && c.fourier <= 2 && (c.dft_cg == 4 || c.dft_cg == 8) && k.K >= 1;
 }
 }  // namespace
@@ -1612,7 +1644,7 @@ int sht_plan_autotune(sht_plan *plan, int32_t K, void *stream) {
         if (!rc) rc = apply_choice(plan, best);
         best_ms = 1e30f;
         TuneChoice b2 = best;
-        for (int xcd = 1; xcd >= 0 && !rc; --xcd)
+        for (int xcd = 3; xcd >= 0 && !rc; --xcd)
             for (int pair = 1; pair >= 0 && !rc; --pair) {
                 if (pair == 0 && best.fourier != sht::kFourierRadix) continue;  // the DFT has one order
                 TuneChoice c = best;
@@ -1643,7 +1675,7 @@ int sht_plan_autotune(sht_plan *plan, int32_t K, void *stream) {
 int sht_plan_set_tuning(sht_plan *plan, const int32_t *in) {
     if (!plan || !in) return fail(SHT_ERR_ARG, "bad arguments");
     if ((in[0] != 11 && in[0] != 17) || in[1] < 0 || in[1] > 1 || in[2] < 0 || in[2] > 1 || in[3] < 0 ||
-        in[3] > 1 || in[4] < 0 || in[4] > 1 || in[5] < 0 || in[5] > 2 || (in[6] != 4 && in[6] != 8))
+        in[3] > 3 || in[4] < 0 || in[4] > 1 || in[5] < 0 || in[5] > 2 || (in[6] != 4 && in[6] != 8))
         return fail(SHT_ERR_ARG, "tuning values out of range");
     FFTPolicy pol;
     pol.mfma_min = in[0];

@@ -72,11 +72,13 @@ class SpectralTransform:
     FOURIER = {"radix": 0, "dft32": 1, "dft16": 2}
 
     def set_tuning(self, mfma_min=17, eights=True, odd_first=False, legendre_xcd_groups=True, fft_pairs=True,
-                   fourier="radix", dft_cols=4):
+                   fourier="radix", dft_cols=4, legendre_light_first=False):
         """Pins the Fourier kernel (the LDS FFT with its factorisation policy, or the ring
-        DFT on MFMA with dft_cols columns per workgroup) and the launch orders
-        (sht_plan_set_tuning)."""
-        v = (C.c_int32 * 7)(int(mfma_min), int(bool(eights)), int(bool(odd_first)), int(bool(legendre_xcd_groups)),
+        DFT on MFMA with dft_cols columns per workgroup) and the launch orders: Legendre tiles
+        of one order grouped on an XCD, the lightest tiles beyond the resident workgroups
+        launched first, FFT column groups paired (sht_plan_set_tuning)."""
+        leg = int(bool(legendre_xcd_groups)) | (2 if legendre_light_first else 0)
+        v = (C.c_int32 * 7)(int(mfma_min), int(bool(eights)), int(bool(odd_first)), leg,
                             int(bool(fft_pairs)), self.FOURIER[fourier], int(dft_cols))
         check(lib().sht_plan_set_tuning(self._plan, v), "sht_plan_set_tuning")
 
@@ -85,8 +87,8 @@ class SpectralTransform:
         check(lib().sht_plan_tuning(self._plan, out), "sht_plan_tuning")
         kinds = {v: k for k, v in self.FOURIER.items()}
         return {"tuned": bool(out[0]), "mfma_min": out[1], "eights": bool(out[2]), "odd_first": bool(out[3]),
-                "legendre_xcd_groups": bool(out[4]), "fft_pairs": bool(out[5]), "fourier": kinds[out[6]],
-                "dft_cols": out[7]}
+                "legendre_xcd_groups": bool(out[4] & 1), "legendre_light_first": bool(out[4] & 2),
+                "fft_pairs": bool(out[5]), "fourier": kinds[out[6]], "dft_cols": out[7]}
 
     def close(self):
         if getattr(self, "_plan", None) and self._plan.value:

@@ -57,3 +57,24 @@ def test_autotune_choice_cached_and_consistent(sw, gpu):
     assert t2 == t
     S.close()
     S2.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16x3", "bf16x3"])
+def test_legendre_launch_orders_bitwise_equal(sw, gpu, precision):
+    """The four Legendre launch orders (XCD grouping, lightest tiles first) run the same
+    tiles, so both transforms come out bitwise equal."""
+    o, a, layers, fields = case(sw, "T255", seed=3)
+    S = fresh_plan(sw, "T255", precision)
+    coeffs = to_dev(sw, a, S.spectrum)
+    outs = []
+    for xcd in (False, True):
+        for light in (False, True):
+            S.set_tuning(17, False, True, xcd, True, legendre_light_first=light)
+            t = S.tuning()
+            assert t["legendre_xcd_groups"] == xcd and t["legendre_light_first"] == light
+            f = sw.transform(coeffs, S)
+            b = sw.transform(f, S)
+            outs.append((f.data.cpu().numpy(), b.data.cpu().numpy()))
+    for f, b in outs[1:]:
+        assert np.array_equal(f, outs[0][0]) and np.array_equal(b, outs[0][1])
+    S.close()
