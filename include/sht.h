@@ -245,8 +245,8 @@ int sht_mlp_destroy(sht_mlp *mlp);
 
 /* Autotuner (no reference counterpart; kernel, tile and order choices per size class): on K
  * columns of scratch data, times the Fourier stage's kernels -- the LDS ring FFT under each
- * factorisation policy (first-pass MFMA threshold 11 or 17, powers of two as 8s or 4s, small
- * odd radices first or last) and the ring DFT on MFMA (FP32, or FP16 hi/lo on split plans)
+ * factorisation policy (first-pass MFMA threshold 11 or 17, powers of two as 4s, 8s or 16s,
+ * small odd radices first or last) and the ring DFT on MFMA (FP32, or FP16 hi/lo on split plans)
  * with 4 or 8 columns per workgroup -- then the launch orders (Legendre tiles grouped per
  * order m on one XCD or plain heaviest-first, each with or without the lightest tiles first;
  * LDS-FFT synthesis column groups paired or not),
@@ -268,7 +268,8 @@ int sht_plan_tuning(const sht_plan *plan, int32_t *out);
  * null).  SHT_ERR_ARG on unreadable files or malformed lines (the valid lines still load). */
 int sht_tune_cache_save(const char *path);
 int sht_tune_cache_load(const char *path, int32_t *nloaded);
-/* pins a choice: in[7] = {MFMA threshold (11 or 17), eights, odd_first, Legendre launch order
+/* pins a choice: in[7] = {MFMA threshold (11 or 17), powers of two as 4s / 8s / 16s (eights =
+ * 0 / 1 / 2), odd_first, Legendre launch order
  * (0..3, bits as above), FFT pairing, Fourier kernel (SHT_FOURIER_*), DFT columns per workgroup
  * (4 or 8)}.  Every choice's tables are built at plan creation, so this
  * changes host state only: calls already queued keep what they were launched with.  Not

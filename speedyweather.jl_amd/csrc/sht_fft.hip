@@ -213,12 +213,13 @@ __device__ inline void bfly_any(float2 *a, const float2 *tw, int stepR) {
 // thread reads all its items into registers, the workgroup synchronises, then the results
 // are written in place.  Columns past the call's last column hold scratch values that are
 // transformed along and never stored.
-// Radices of 11 and more (one column per item) run as two sub-passes of two columns each,
-// so that a thread holds at most one butterfly.
+// Radices 11 and 13 (one column per item) run as two sub-passes of two columns each, so
+// that a thread holds at most one butterfly; radix 16 (N / 16 butterflies per column, at
+// most 256 items over four columns) needs no split.
 template <int R>
 struct PassCols {
     static constexpr int CB = R <= 7 ? 4 : R <= 8 ? 2 : 1;  // columns per work item
-    static constexpr int SUB = R <= 8 ? 1 : 2;              // column sub-passes
+    static constexpr int SUB = R <= 8 || R == 16 ? 1 : 2;   // column sub-passes
     static constexpr int NCG = kFFTCols / CB / SUB;         // column groups per sub-pass
     static constexpr int MAXI = (kFFTMaxRing / R * NCG + kFFTThreads - 1) / kFFTThreads;
 };
@@ -288,216 +289,136 @@ __device__ void fft_pass_wg(float2 *buf, const float2 *tw, int N, const FFTPass 
         fft_subpass_wg<R, INV>(buf + sp * (kFFTCols / PassCols<R>::SUB) * N, tw, N, pd, tid);
 }
 
-// First pass (l = 1) of an odd prime radix r >= 17 as two small GEMMs on MFMA, in place.
+// First pass (l = 1) of an odd prime radix r >= 11 as two small GEMMs on MFMA, in place.
 // With the inputs x_q = buf[b + M q] of butterfly b, a_t = x_t + x_{r-t},
 // b_t = x_t - x_{r-t} (t = 1..h, h = (r-1)/2):
 //   T1[s] = sum_t cos(2 pi s t / r) a_t   (s = 0..h),   T2[s] = sum_t sin(2 pi s t / r) b_t
 //   y_0 = x_0 + T1[0],  y_s = x_0 + T1[s] -+ i T2[s],  y_{r-s} = x_0 + T1[s] +- i T2[s]
 // (upper signs forward), y_s -> buf[b + M s].  GEMM rows = s (row tiles of 16), columns =
-// (butterfly, re/im) (column tiles of 16), K = t (steps of 4).
+// (butterfly, re/im) (column tiles of 16), K = t in chunks of 32.
 //
-// Each wave owns one or two row tiles (or, with fewer than four row tiles, a share of the
-// column tiles of one row tile) and sweeps its column tiles in rounds of MT x NT tiles, so
-// one cos/sin operand serves NT column tiles per K-step.  The cos/sin operand of lane
-// (row s, t) is the ring's twiddle w_N^{(s t mod r) M} from LDS, its index advanced by
-// 4 s mod r per K-step; all operands are loaded two K-steps ahead of their MFMAs.
-// The B operands are read as x_t, x_{r-t} and combined into a_t, b_t right before their
-// MFMAs.  Butterflies read and write only their own r slots, so all tiles of a round are
-// computed before any of their outputs is written.
-template <int MT, int NT>
-struct MfmaRound {
-    f4 t1[MT][NT], t2[MT][NT];
-};
-
-template <bool INV, int MT, int NT>
-__device__ inline void mfma_prime_rounds(float2 *buf, const float2 *tw, int N, const FFTPass &pd, int h, int nseq,
-                                         int mt0, int mstep, int nmt, int nt0, int ntstep, int ntround, int Nt,
-                                         int lane) {
-    const int r = pd.R, M = pd.M;
-    const int row = lane & 15, grp = lane >> 4;
-    const int Kt = (h + 3) >> 2;
-    const int t0 = grp + 1;
-    const FastDiv divM(M, pd.mulM), divR(r, pd.mulR);
-    const float *bf = reinterpret_cast<const float *>(buf);
-    // cos/sin operand state per owned row tile
-    int e0[MT], de[MT];
+// Products in three-part BF16 (v_mfma_f32_16x16x32_bf16, 16 cycles per 32-wide K chunk,
+// against 8 FP32 MFMAs 16x16x4 of 32 cycles): cos / sin come from the plan's table for r
+// (hi + mid + lo of the FP64 values, sht_plan.cpp: prime_tables); a_t and b_t are split the
+// same way once per chunk and wave (x = hi + mid + lo exactly: BF16 keeps FP32's exponent
+// range, so no scaling); the six products whose parts sum to at most 2 accumulate in FP32,
+// smallest first, so the pass keeps FP32-level accuracy in every precision mode.
+//
+// Column tiles go to the waves in rounds of four; with fewer column tiles left, the waves of
+// one column tile split its row tiles (at most kPrimeRowTiles per wave: rings of N = 4p
+// points have M = N / r >= 4, see sht_plan.cpp: factorize).  All row tiles of a round's
+// column tiles are computed before the workgroup synchronises and any output is written
+// (a butterfly's outputs overwrite its inputs, which the other row tiles still read).
+// tmax: the last pair t with nonzero inputs (a synthesis reads only the orders m <= mt and
+// their mirrors N - m: x_t, x_{r-t} vanish for t > (mt + M - 1) / M); h otherwise.
+__device__ inline void split3_bf16(const float (&x)[8], b8 &hi, b8 &mid, b8 &lo) {
 #pragma unroll
-    for (int m = 0; m < MT; ++m) {
-        const int s = min((mt0 + m * mstep) * 16 + row, r - 1);  // rows above h are discarded
-        e0[m] = (s * t0 - divR.div(s * t0) * r) * M;
-        de[m] = (4 * s - divR.div(4 * s) * r) * M;
+    for (int e = 0; e < 8; ++e) {
+        const __bf16 h = (__bf16)x[e];
+        const float r1 = x[e] - (float)h;  // exact
+        const __bf16 m = (__bf16)r1;
+        hi[e] = h;
+        mid[e] = m;
+        lo[e] = (__bf16)(r1 - (float)m);
     }
-    const bool last_ok = t0 + 4 * (Kt - 1) <= h;  // this lane's t in the last K-step is used
-    for (int rb = 0; rb * ntround < Nt; ++rb) {
-        int pa[NT], pb[NT];  // float offsets of this lane's a_t, b_t in buf
+}
+
+// c += a b over the parts (i, j) with i + j <= 2, smallest first
+__device__ inline f4 mma_bf16x3(const b8 &ah, const b8 &am, const b8 &al, const b8 &bh, const b8 &bm, const b8 &bl,
+                                f4 c) {
+    c = Split<kBF16x3>::mma(al, bh, c);
+    c = Split<kBF16x3>::mma(ah, bl, c);
+    c = Split<kBF16x3>::mma(am, bm, c);
+    c = Split<kBF16x3>::mma(am, bh, c);
+    c = Split<kBF16x3>::mma(ah, bm, c);
+    return Split<kBF16x3>::mma(ah, bh, c);
+}
+
+template <bool INV>
+__device__ void fft_pass_mfma_first(float2 *buf, const SHT_GLOBAL b8 *tab, int N, const FFTPass &pd, int ncol,
+                                    int tmax, int tid) {
+    const int r = pd.R, M = pd.M;
+    const int h = (r - 1) >> 1;
+    const int nseq = ncol * M;  // butterflies over all columns
+    constexpr int NW = kFFTThreads / 64;
+    // the wave index is uniform per wave: keep it (and all tile ownership) scalar
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int row = lane & 15, grp = lane >> 4, ri = row & 1;
+    const int Mt = (h + 16) >> 4;         // row tiles (rows 0..h)
+    const int Nt = (2 * nseq + 15) >> 4;  // column tiles over (butterfly, re/im)
+    const int KC = (h + 31) >> 5;         // K chunks of the table
+    const int KU = (tmax + 31) >> 5;      // K chunks with nonzero inputs
+    const FastDiv divM(M, pd.mulM);
+    const float *bf = reinterpret_cast<const float *>(buf);
+    for (int n0 = 0; n0 < Nt; n0 += NW) {  // uniform: every wave takes part in every round's barriers
+        const int nc = min(NW, Nt - n0);   // column tiles this round
+        const int G = NW / nc;             // waves per column tile
+        const int nt = n0 + wave % nc, r0 = wave / nc;
+        const int nrt = wave < nc * G ? min((Mt - r0 + G - 1) / G, kPrimeRowTiles) : 0;  // uniform per wave
+        f4 t1[kPrimeRowTiles], t2[kPrimeRowTiles];
 #pragma unroll
-        for (int n = 0; n < NT; ++n) {
-            const int nt = nt0 + rb * ntround + n * ntstep;
+        for (int i = 0; i < kPrimeRowTiles; ++i) t1[i] = t2[i] = f4{0.f, 0.f, 0.f, 0.f};
+        int ob = -1;
+        float2 z0 = make_float2(0.f, 0.f);
+        if (nrt > 0) {
             const int c = nt * 16 + row, sq = c >> 1;
             int base = 0;  // columns past the data read column 0; their outputs are dropped
             if (sq < nseq) {
                 const int col = divM.div(sq);
                 base = col * N + (sq - col * M);
+                ob = base;
+                z0 = buf[base];  // x_0, read before any output of the round is written
             }
-            pa[n] = 2 * base + (c & 1) + 2 * M * t0;
-            pb[n] = 2 * base + (c & 1) + 2 * M * (r - t0);
-        }
-        int e[MT];
-        MfmaRound<MT, NT> acc;
+            for (int kc = 0; kc < KU; ++kc) {
+                float xs[8], xd[8];
 #pragma unroll
-        for (int m = 0; m < MT; ++m) {
-            e[m] = e0[m];
-#pragma unroll
-            for (int n = 0; n < NT; ++n) {
-                acc.t1[m][n] = f4{0.f, 0.f, 0.f, 0.f};
-                acc.t2[m][n] = f4{0.f, 0.f, 0.f, 0.f};
-            }
-        }
-        // Operands (twiddles; x_t in X, x_{r-t} in Y) of even K-steps in set 0, odd ones in
-        // set 1; a set is refilled with the step two ahead right after its MFMAs, so loads
-        // are always two steps in flight.
-        // Fetches past the last step re-read the last step's operands (never used): the
-        // loads stay unconditional and the wait for a set never waits for the other one.
-        struct Ops {
-            float2 A[MT];
-            float X[NT], Y[NT];
-        } S0, S1;
-        int kf = 0;  // next step to fetch
-        auto fetch = [&](Ops &S) {
-#pragma unroll
-            for (int m = 0; m < MT; ++m) S.A[m] = tw[e[m]];
-#pragma unroll
-            for (int n = 0; n < NT; ++n) {
-                S.X[n] = bf[pa[n]];
-                S.Y[n] = bf[pb[n]];
-            }
-            const bool adv = ++kf < Kt;  // uniform
-            if (adv) {
-#pragma unroll
-                for (int m = 0; m < MT; ++m) {
-                    e[m] += de[m];
-                    e[m] -= e[m] >= N ? N : 0;
+                for (int e = 0; e < 8; ++e) {
+                    const int t = 32 * kc + 8 * grp + e + 1;
+                    const bool ok = t <= tmax;
+                    const int tt = ok ? t : tmax;
+                    const float xa = bf[2 * (base + M * tt) + ri], xb = bf[2 * (base + M * (r - tt)) + ri];
+                    xs[e] = ok ? xa + xb : 0.f;
+                    xd[e] = ok ? xa - xb : 0.f;
                 }
+                b8 sh, sm, sl, dh, dm, dl;
+                split3_bf16(xs, sh, sm, sl);
+                split3_bf16(xd, dh, dm, dl);
 #pragma unroll
-                for (int n = 0; n < NT; ++n) {
-                    pa[n] += 8 * M;
-                    pb[n] -= 8 * M;
-                }
-            }
-        };
-        auto mma = [&](const Ops &S, int ks) {
-            const bool mask = ks == Kt - 1 && !last_ok;  // this lane's t is past h
-#pragma unroll
-            for (int m = 0; m < MT; ++m) {
-                if (m >= nmt) break;  // uniform per wave
-                // the table holds w^k = cos - i sin (forward twiddles)
-                const float cs = mask ? 0.f : S.A[m].x, sn = mask ? 0.f : -S.A[m].y;
-#pragma unroll
-                for (int n = 0; n < NT; ++n) {
-                    acc.t1[m][n] = mfma_f32(cs, S.X[n] + S.Y[n], acc.t1[m][n]);  // a_t
-                    acc.t2[m][n] = mfma_f32(sn, S.X[n] - S.Y[n], acc.t2[m][n]);  // b_t
-                }
-            }
-        };
-        fetch(S0);
-        fetch(S1);
-        int ks = 0;
-        for (; ks + 1 < Kt; ks += 2) {  // straight-line body: the waits stay per set
-            mma(S0, ks);
-            fetch(S0);
-            mma(S1, ks + 1);
-            fetch(S1);
-        }
-        if (ks < Kt) mma(S0, ks);
-        // x_0 of each column's butterfly, read before any output of the round is written
-        int ob[NT];
-        float2 z0[NT];
-#pragma unroll
-        for (int n = 0; n < NT; ++n) {
-            const int nt = nt0 + rb * ntround + n * ntstep;
-            const int sq = (nt * 16 + row) >> 1;
-            ob[n] = -1;
-            z0[n] = make_float2(0.f, 0.f);
-            if (nt < Nt && sq < nseq) {
-                const int col = divM.div(sq);
-                ob[n] = col * N + (sq - col * M);
-                z0[n] = buf[ob[n]];
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < MT; ++m) {
-            if (m >= nmt) break;  // uniform per wave
-            const int mt = mt0 + m * mstep;
-#pragma unroll
-            for (int n = 0; n < NT; ++n) {
-                const int reim = row & 1;
-                if (ob[n] < 0) continue;
-                const float2 z = z0[n];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const float a1 = acc.t1[m][n][i], a2 = acc.t2[m][n][i];
-                    const float p1 = lane_xor1(a1), p2 = lane_xor1(a2);
-                    const int s = mt * 16 + 4 * grp + i;
-                    if (s > h) continue;
-                    const float T1re = reim ? p1 : a1, T1im = reim ? a1 : p1;
-                    const float T2re = reim ? p2 : a2, T2im = reim ? a2 : p2;
-                    const float sg = INV ? -1.f : 1.f;
-                    if (s == 0) {
-                        if (reim == 0) buf[ob[n]] = make_float2(z.x + T1re, z.y + T1im);
-                    } else if (reim == 0) {
-                        buf[ob[n] + M * s] = make_float2(z.x + T1re + sg * T2im, z.y + T1im - sg * T2re);
-                    } else {
-                        buf[ob[n] + M * (r - s)] = make_float2(z.x + T1re - sg * T2im, z.y + T1im + sg * T2re);
-                    }
+                for (int i = 0; i < kPrimeRowTiles; ++i) {
+                    if (i >= nrt) break;  // uniform per wave
+                    const SHT_GLOBAL b8 *a = tab + ((long long)((r0 + i * G) * KC + kc) * 64 + lane) * 6;
+                    const b8 ch = a[0], cm = a[1], cl = a[2];
+                    t1[i] = mma_bf16x3(ch, cm, cl, sh, sm, sl, t1[i]);
+                    const b8 nh = a[3], nm = a[4], nl = a[5];
+                    t2[i] = mma_bf16x3(nh, nm, nl, dh, dm, dl, t2[i]);
                 }
             }
         }
         __syncthreads();
-    }
-}
-
-template <bool INV>
-__device__ void fft_pass_mfma_first(float2 *buf, const float2 *tw, int N, const FFTPass &pd, int ncol, int tid) {
-    const int h = (pd.R - 1) >> 1;
-    const int nseq = ncol * pd.M;  // butterflies over all columns
-    constexpr int NW = kFFTThreads / 64;
-    // the wave index is uniform per wave: keep it (and all tile ownership) scalar
-    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int Mt = (h + 16) >> 4;         // row tiles (rows 0..h), at most 2 NW
-    const int Nt = (2 * nseq + 15) >> 4;  // column tiles over (butterfly, re/im)
-    if (Nt <= 2 && Mt <= NW) {
-        // one or two column tiles (rings of nlon = 4 r) and at most NW row tiles: each wave
-        // takes one column tile and every (NW / Nt)-th row tile, so no MFMA runs on padding
-        // columns; one round
-        const int share = NW / Nt;  // waves per column tile
-        const int col = Nt == 2 ? (wave & 1) : 0;
-        const int r0 = Nt == 2 ? (wave >> 1) : wave;
-        const int nmt = r0 < Mt ? (Mt - r0 + share - 1) / share : 0;
-        if (nmt > 0) {
-            mfma_prime_rounds<INV, 4, 1>(buf, tw, N, pd, h, nseq, r0, share, nmt, col, 1, 2, Nt, lane);
-        } else {  // idle wave: the round's two barriers
-            __syncthreads();
-            __syncthreads();
-        }
-    } else if (Mt > NW) {
-        // waves own row tiles wave and wave + NW; rounds of 2 column tiles
-        const int nmt = wave + NW < Mt ? 2 : 1;
-        mfma_prime_rounds<INV, 2, 2>(buf, tw, N, pd, h, nseq, wave, NW, nmt, 0, 1, 2, Nt, lane);
-    } else {
-        // G = NW / Mt waves share a row tile, each taking every G-th column tile
-        const int G = NW / Mt;
-        const int grp = wave / Mt;
-        if (grp < G) {
-            mfma_prime_rounds<INV, 1, 4>(buf, tw, N, pd, h, nseq, wave % Mt, 0, 1, grp, G, 4 * G, Nt, lane);
-        } else {
-            // idle wave: still takes part in every round's two barriers
-            const int rounds = (Nt + 4 * G - 1) / (4 * G);
-            for (int rb = 0; rb < rounds; ++rb) {
-                __syncthreads();
-                __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kPrimeRowTiles; ++i) {
+            if (i >= nrt) break;  // uniform per wave
+            const int mt = r0 + i * G;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                // the partner lane (c ^ 1) holds the other part of the same butterfly
+                const float a1 = t1[i][e], a2 = t2[i][e];
+                const float p1 = lane_xor1(a1), p2 = lane_xor1(a2);
+                const int s = mt * 16 + 4 * grp + e;
+                if (ob < 0 || s > h) continue;
+                const float T1re = ri ? p1 : a1, T1im = ri ? a1 : p1;
+                const float T2re = ri ? p2 : a2, T2im = ri ? a2 : p2;
+                const float sg = INV ? -1.f : 1.f;
+                if (s == 0) {
+                    if (ri == 0) buf[ob] = make_float2(z0.x + T1re, z0.y + T1im);
+                } else if (ri == 0) {
+                    buf[ob + M * s] = make_float2(z0.x + T1re + sg * T2im, z0.y + T1im - sg * T2re);
+                } else {
+                    buf[ob + M * (r - s)] = make_float2(z0.x + T1re - sg * T2im, z0.y + T1im + sg * T2re);
+                }
             }
         }
+        __syncthreads();
     }
 }
 
@@ -516,17 +437,25 @@ __device__ __forceinline__ FFTPass load_pass(const SHT_GLOBAL FFTPass *p) {
 // Runs all passes in place on buf (kFFTCols columns of N points, the first ncol of them
 // data): a first-pass MFMA radix, then the register passes, all across the workgroup.
 template <bool INV>
-__device__ __attribute__((always_inline)) void fft_run(float2 *buf, const float2 *tw, const SHT_GLOBAL FFTPass *passes,
-                                                        int nf, int N, int ncol, int tid) {
+__device__ __attribute__((always_inline)) void fft_run(const DevPlan &P, float2 *buf, const float2 *tw,
+                                                        const SHT_GLOBAL FFTPass *passes, int nf, int N, int mt,
+                                                        int ncol, int tid) {
     int f = 0;
-    if (nf > 0 && load_pass(passes).R >= kMfmaMinPrime) {
+    // an odd first radix of 11 or more is a prime the plan placed first for the MFMA pass
+    // (register passes of 11 and 13 come last; 16 is the only even register radix above 8)
+    if (nf > 0 && load_pass(passes).R >= kMfmaMinPrime && (load_pass(passes).R & 1)) {
         const FFTPass p0 = load_pass(passes);
-        fft_pass_mfma_first<INV>(buf, tw, N, p0, ncol, tid);
+        const SHT_GLOBAL b8 *tab = reinterpret_cast<const SHT_GLOBAL b8 *>(as_global(P.prime_tab)) +
+                                   as_global(P.prime_off)[p0.R];
+        const int h = (p0.R - 1) >> 1;
+        const int tmax = INV ? min(h, (mt + p0.M - 1) / p0.M) : h;
+        fft_pass_mfma_first<INV>(buf, tab, N, p0, ncol, max(tmax, 1), tid);
         f = 1;
     }
     for (int g = f; g < nf; ++g) {
         const FFTPass pd = load_pass(passes + g);
         switch (pd.R) {
+            case 16: fft_pass_wg<16, INV>(buf, tw, N, pd, tid); break;
             case 8: fft_pass_wg<8, INV>(buf, tw, N, pd, tid); break;
             case 2: fft_pass_wg<2, INV>(buf, tw, N, pd, tid); break;
             case 3: fft_pass_wg<3, INV>(buf, tw, N, pd, tid); break;
@@ -669,7 +598,7 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_synthesis_kernel(const Dev
         }
     }
     __syncthreads();
-    fft_run<true>(b0, tw, as_global(P.passes) + G.j * kMaxFactors, as_global(P.nfac)[G.j], N, ncol, tid);
+    fft_run<true>(P, b0, tw, as_global(P.passes) + G.j * kMaxFactors, as_global(P.nfac)[G.j], N, mt, ncol, tid);
     __syncthreads();
     {
         SHT_GLOBAL float *field = as_global(static_cast<float *>(C.out));
@@ -767,7 +696,7 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_analysis_kernel(const DevP
         }
     }
     __syncthreads();
-    fft_run<false>(b0, tw, as_global(P.passes) + G.j * kMaxFactors, as_global(P.nfac)[G.j], N, ncol, tid);
+    fft_run<false>(P, b0, tw, as_global(P.passes) + G.j * kMaxFactors, as_global(P.nfac)[G.j], N, mt, ncol, tid);
     __syncthreads();
     {
         SHT_GLOBAL float4 *fb = as_global(static_cast<float4 *>(C.fbuf)) + as_global(P.fboff)[G.j] * (long long)K + G.k0;

@@ -47,11 +47,22 @@ constexpr int kFFTCols = 4;
 constexpr int kMfmaMinPrime = 11;
 constexpr int kMfmaMinRadix = 17;
 constexpr int kMaxRegisterRadix = 13;
-// MFMA GEMM pass of the FFT: tiles (16 output frequencies x 16 real columns) a wave keeps
-// in flight within one round; rows up to 16 * kMfmaTilesPerWave * waves, i.e. radices up
-// to kMaxMfmaRadix (255).
-constexpr int kMfmaTilesPerWave = 2;
-constexpr int kMaxMfmaRadix = 2 * 16 * kMfmaTilesPerWave * (kFFTThreads / 64) - 1;
+// MFMA prime pass of the FFT: largest radix (its tables are indexed by the radix) and the
+// row tiles (16 output frequencies each) one wave holds for its column tile in a round.
+constexpr int kMaxMfmaRadix = 255;
+constexpr int kPrimeRowTiles = 4;
+// Row tiles per wave of the MFMA prime pass for radix r on M = N / r butterflies per column
+// and ncol columns, maximised over the rounds (host check of kPrimeRowTiles).
+inline int prime_rows_per_wave(int r, int M, int ncol) {
+    const int Mt = ((r - 1) / 2 + 16) / 16, Nt = (2 * ncol * M + 15) / 16, NW = kFFTThreads / 64;
+    int worst = 0;
+    for (int n0 = 0; n0 < Nt; n0 += NW) {
+        const int nc = Nt - n0 < NW ? Nt - n0 : NW, G = NW / nc;
+        const int rows = (Mt + G - 1) / G;
+        worst = rows > worst ? rows : worst;
+    }
+    return worst;
+}
 
 enum Mode { kFP32 = 0, kFP16x3 = 1, kBF16x3 = 2 };
 // 16-bit parts per operand value: scaled FP16 hi + lo (three products), BF16 hi + mid + lo
@@ -204,6 +215,12 @@ struct DevPlan {
     const long long *tw_off;
     const FFTPass *passes;  // kMaxFactors passes per ring, in order
     const int *nfac;
+    // First-pass prime radices r (MFMA pass): per r, cos / sin(2 pi s t / r) for rows
+    // s = 0..h (tiles of 16) and t = 1..h (chunks of 32), h = (r-1)/2, as three BF16 parts of
+    // the FP64 values in MFMA fragment order: per (row tile, chunk, lane) six vectors of 8
+    // (cos hi, mid, lo, sin hi, mid, lo).  prime_off[r] (r < 256): r's first vector.
+    const unsigned short *prime_tab;
+    const long long *prime_off;
     // Ring DFT on MFMA (rings of N = 4p points): per ring, the cos / sin matrices
     // cos(2 pi r t / p), sin(2 pi r t / p), rows r = 0..p/2 (tiles of 16), columns
     // t = 1..(p-1)/2, in MFMA fragment order: FP32 (dft32, chunks of 16 t, a float4 of cos

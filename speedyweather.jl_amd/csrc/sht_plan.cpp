@@ -252,18 +252,18 @@ struct Profile {
 // Ring FFT factorisation policy (a tuning knob of the autotuner).
 struct FFTPolicy {
     int mfma_min = 17;       // a largest prime >= this runs first, as the MFMA pass
-    bool eights = true;      // powers of two as 8s (then one 4 or 2) rather than 4s
+    int eights = 1;          // powers of two as 4s (0), 8s (1) or 16s (2), then the rest as 8, 4, 2
     bool odd_first = false;  // small odd radices before the powers of two
 };
 // Every policy's FFT tables live on the device from plan creation on, so choosing one
 // (autotuning, pinning) changes host state only and never rewrites memory that kernels in
 // flight on other streams may read.
-constexpr int kPolicies = 8;
-inline int policy_index(const FFTPolicy &p) { return (p.mfma_min == 11 ? 4 : 0) + (p.eights ? 2 : 0) + (p.odd_first ? 1 : 0); }
+constexpr int kPolicies = 12;
+inline int policy_index(const FFTPolicy &p) { return (p.mfma_min == 11 ? 6 : 0) + 2 * p.eights + (p.odd_first ? 1 : 0); }
 inline FFTPolicy policy_of(int i) {
     FFTPolicy p;
-    p.mfma_min = (i & 4) ? 11 : 17;
-    p.eights = (i & 2) != 0;
+    p.mfma_min = i >= 6 ? 11 : 17;
+    p.eights = (i % 6) / 2;
     p.odd_first = (i & 1) != 0;
     return p;
 }
@@ -464,12 +464,17 @@ int factorize(int N, int *fac, const FFTPolicy &pol) {
     size_t used = 0;
     if (!big.empty() && big[0] >= std::min(pol.mfma_min, sht::kMfmaMinRadix)) {
         if (big[0] > sht::kMaxMfmaRadix) return -1;
+        for (int ncol = 1; ncol <= sht::kFFTCols; ++ncol)
+            if (sht::prime_rows_per_wave(big[0], N / big[0], ncol) > sht::kPrimeRowTiles) return -1;
         fac[nf++] = big[0];
         r /= big[0];
         used = 1;
     }
     auto pow2 = [&]() {
-        if (pol.eights) {
+        if (pol.eights == 2) {
+            while (r % 16 == 0) { fac[nf++] = 16; r /= 16; }
+        }
+        if (pol.eights >= 1) {
             while (r % 8 == 0) { fac[nf++] = 8; r /= 8; }
         }
         while (r % 4 == 0) { fac[nf++] = 4; r /= 4; }
@@ -523,7 +528,7 @@ int fft_tables(const Geometry &g, const FFTPolicy &pol, std::vector<int> &fac, s
         double c = 0;
         for (int f = 0; f < nfac[j]; ++f) {
             const int r = fac[(size_t)j * sht::kMaxFactors + f];
-            c += r <= 8 ? 1.0 : (f == 0 ? 0.25 * r : 0.5 * r);  // MFMA first pass or register butterflies
+            c += r <= 16 ? 1.0 : (f == 0 ? 0.25 * r : 0.5 * r);  // MFMA first pass or register butterflies
         }
         cost[j] = c * g.nlon[j];
     }
@@ -531,6 +536,52 @@ int fft_tables(const Geometry &g, const FFTPolicy &pol, std::vector<int> &fac, s
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
     return SHT_OK;
+}
+
+// Tables of the MFMA prime pass (see DevPlan::prime_tab) for every radix r >= 11 that is
+// the first pass of some ring under some factorisation policy.
+void prime_tables(const std::vector<int> (&fac_v)[kPolicies], const std::vector<int> (&nfac_v)[kPolicies],
+                  const bool (&ok)[kPolicies], int n, std::vector<uint16_t> &tab, std::vector<long long> &off) {
+    std::vector<bool> used(sht::kMaxMfmaRadix + 1, false);
+    for (int q = 0; q < kPolicies; ++q) {
+        if (!ok[q]) continue;
+        for (int j = 0; j < n; ++j) {
+            if (nfac_v[q][j] < 1) continue;
+            const int r = fac_v[q][(size_t)j * sht::kMaxFactors];
+            if (r >= sht::kMfmaMinPrime && r <= sht::kMaxMfmaRadix && (r & 1)) used[r] = true;
+        }
+    }
+    off.assign(sht::kMaxMfmaRadix + 1, 0);
+    tab.clear();
+    for (int r = sht::kMfmaMinPrime; r <= sht::kMaxMfmaRadix; ++r) {
+        if (!used[r]) continue;
+        const int h = (r - 1) / 2, Mt = (h + 16) / 16, KC = (h + 31) / 32;
+        off[r] = (long long)(tab.size() / 8);
+        tab.resize(tab.size() + (size_t)Mt * KC * 64 * 6 * 8, 0);
+        for (int mt = 0; mt < Mt; ++mt)
+            for (int kc = 0; kc < KC; ++kc)
+                for (int lane = 0; lane < 64; ++lane)
+                    for (int e = 0; e < 8; ++e) {
+                        const int s = 16 * mt + lane % 16, t = 32 * kc + 8 * (lane / 16) + e + 1;
+                        double v[2] = {0.0, 0.0};
+                        if (s <= h && t <= h) {
+                            const double a = 2.0 * M_PI * (double)(((long long)s * t) % r) / (double)r;
+                            v[0] = std::cos(a);
+                            v[1] = std::sin(a);
+                        }
+                        const size_t base = (size_t)8 * (off[r] + ((size_t)(mt * KC + kc) * 64 + lane) * 6) + e;
+                        for (int cs = 0; cs < 2; ++cs) {
+                            const uint16_t hi = to_bf16(v[cs]);
+                            const double r1 = v[cs] - bf16_value(hi);
+                            const uint16_t mid = to_bf16(r1);
+                            const uint16_t lo = to_bf16(r1 - bf16_value(mid));
+                            tab[base + (size_t)(3 * cs + 0) * 8] = hi;
+                            tab[base + (size_t)(3 * cs + 1) * 8] = mid;
+                            tab[base + (size_t)(3 * cs + 2) * 8] = lo;
+                        }
+                    }
+    }
+    if (tab.empty()) tab.assign(8, 0);
 }
 
 // Matrices of the ring DFT on MFMA (see DevPlan::dft32 / dft16): for every ring of
@@ -852,6 +903,13 @@ int build_plan(sht_plan *pl) {
         if ((rc = dev_upload(A, ana_f32, &P.ana_f32))) return rc;
     }
     if ((rc = dev_upload(A, tw, &P.twiddle))) return rc;
+    {
+        std::vector<uint16_t> ptab;
+        std::vector<long long> poff;
+        prime_tables(fac_v, nfac_v, pl->pol_ok, n, ptab, poff);
+        if ((rc = dev_upload(A, ptab, &P.prime_tab))) return rc;
+        if ((rc = dev_upload(A, poff, &P.prime_off))) return rc;
+    }
     if ((rc = dev_upload(A, tw_off, &P.tw_off))) return rc;
     {
         std::vector<float> t32;
@@ -1523,13 +1581,14 @@ bool tune_from_line(const std::string &line, TuneKey &k, TuneChoice &c) {
     }
     k = TuneKey{(int)v[0], (int)v[1], (int)v[2], (int)v[3], (int)v[4], (int)v[5], (int)v[6]};
     c.pol.mfma_min = (int)v[7];
-    c.pol.eights = v[8] != 0;
+    c.pol.eights = (int)v[8];
     c.pol.odd_first = v[9] != 0;
     c.leg_xcd = (int)v[10];
     c.fft_pair = (int)v[11];
     c.fourier = (int)v[12];
     c.dft_cg = (int)v[13];
-    return (c.pol.mfma_min == 11 || c.pol.mfma_min == 17) && c.leg_xcd >= 0 && c.leg_xcd <= 3 && c.fft_pair >= 0 &&
+    return (c.pol.mfma_min == 11 || c.pol.mfma_min == 17) && c.pol.eights >= 0 && c.pol.eights <= 2 &&
+           c.leg_xcd >= 0 && c.leg_xcd <= 3 && c.fft_pair >= 0 &&
            c.fft_pair <= 1 && c.fourier >= 0 && c.fourier <= 2 && (c.dft_cg == 4 || c.dft_cg == 8) && k.K >= 1;
 }
 }  // namespace
@@ -1674,12 +1733,12 @@ int sht_plan_autotune(sht_plan *plan, int32_t K, void *stream) {
 
 int sht_plan_set_tuning(sht_plan *plan, const int32_t *in) {
     if (!plan || !in) return fail(SHT_ERR_ARG, "bad arguments");
-    if ((in[0] != 11 && in[0] != 17) || in[1] < 0 || in[1] > 1 || in[2] < 0 || in[2] > 1 || in[3] < 0 ||
+    if ((in[0] != 11 && in[0] != 17) || in[1] < 0 || in[1] > 2 || in[2] < 0 || in[2] > 1 || in[3] < 0 ||
         in[3] > 3 || in[4] < 0 || in[4] > 1 || in[5] < 0 || in[5] > 2 || (in[6] != 4 && in[6] != 8))
         return fail(SHT_ERR_ARG, "tuning values out of range");
     FFTPolicy pol;
     pol.mfma_min = in[0];
-    pol.eights = in[1] != 0;
+    pol.eights = in[1];
     pol.odd_first = in[2] != 0;
     return apply_choice(plan, TuneChoice{pol, in[3], in[4], in[5], in[6]});
 }
@@ -1689,7 +1748,7 @@ int sht_plan_tuning(const sht_plan *plan, int32_t *out) {
     out[0] = plan->tuned;
     const FFTPolicy pol = policy_of(plan->pol);
     out[1] = pol.mfma_min;
-    out[2] = pol.eights ? 1 : 0;
+    out[2] = pol.eights;
     out[3] = pol.odd_first ? 1 : 0;
     out[4] = plan->leg_xcd;
     out[5] = plan->fft_pair;

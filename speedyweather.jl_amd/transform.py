@@ -73,12 +73,14 @@ class SpectralTransform:
 
     def set_tuning(self, mfma_min=17, eights=True, odd_first=False, legendre_xcd_groups=True, fft_pairs=True,
                    fourier="radix", dft_cols=4, legendre_light_first=False):
-        """Pins the Fourier kernel (the LDS FFT with its factorisation policy, or the ring
+        """Pins the Fourier kernel (the LDS FFT with its factorisation policy: a first-pass
+        MFMA prime from mfma_min on, powers of two as 4s, 8s or 16s (eights = 0, 1, 2), small
+        odd radices first or last; or the ring
         DFT on MFMA with dft_cols columns per workgroup) and the launch orders: Legendre tiles
         of one order grouped on an XCD, the lightest tiles beyond the resident workgroups
         launched first, FFT column groups paired (sht_plan_set_tuning)."""
         leg = int(bool(legendre_xcd_groups)) | (2 if legendre_light_first else 0)
-        v = (C.c_int32 * 7)(int(mfma_min), int(bool(eights)), int(bool(odd_first)), leg,
+        v = (C.c_int32 * 7)(int(mfma_min), int(eights), int(bool(odd_first)), leg,
                             int(bool(fft_pairs)), self.FOURIER[fourier], int(dft_cols))
         check(lib().sht_plan_set_tuning(self._plan, v), "sht_plan_set_tuning")
 
@@ -86,7 +88,7 @@ class SpectralTransform:
         out = (C.c_int32 * 8)()
         check(lib().sht_plan_tuning(self._plan, out), "sht_plan_tuning")
         kinds = {v: k for k, v in self.FOURIER.items()}
-        return {"tuned": bool(out[0]), "mfma_min": out[1], "eights": bool(out[2]), "odd_first": bool(out[3]),
+        return {"tuned": bool(out[0]), "mfma_min": out[1], "eights": out[2], "odd_first": bool(out[3]),
                 "legendre_xcd_groups": bool(out[4] & 1), "legendre_light_first": bool(out[4] & 2),
                 "fft_pairs": bool(out[5]), "fourier": kinds[out[6]], "dft_cols": out[7]}
 

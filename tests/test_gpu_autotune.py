@@ -12,7 +12,7 @@ from test_gpu_transform import case, to_dev
 
 pytestmark = pytest.mark.gpu
 
-POLICIES = list(itertools.product([17, 11], [True, False], [False, True]))
+POLICIES = list(itertools.product([17, 11], [0, 1, 2], [False, True]))
 
 
 def fresh_plan(sw, name, precision="fp32"):
