@@ -305,12 +305,16 @@ def profile_table(name):
 def measured_traffic(kernel, precision):
     """HBM bytes per launch (fetch + write) of `kernel` from the committed rocprofv3
     counter summary (FETCH_SIZE / WRITE_SIZE, scaled by a plain copy), or None."""
-    r = profile_table("traffic_t255_k64.txt").get((kernel, precision))
-    return int(r[0]) + int(r[1]) if r else None
+    r = profile_table("counters_t255_k64.txt").get((kernel, precision))
+    try:
+        return int(r[0]) + int(r[1]) if r else None
+    except ValueError:  # a pass without FETCH_SIZE / WRITE_SIZE
+        return None
 
 
 def profiled_ms(kernel, precision):
-    """Mean GPU duration (ms) of `kernel` in the committed rocprofv3 kernel-trace summary."""
+    """Mean GPU duration (ms) of back-to-back launches of `kernel` in the committed rocprofv3
+    kernel-trace summary (the launches kernel_ms times)."""
     r = profile_table("kernel_times_t255_k64.txt").get((kernel, precision))
     return float(r[0]) * 1e-3 if r else None
 
@@ -321,18 +325,25 @@ MODE_NAMES = {"0": "fp32", "1": "fp16x3", "2": "bf16x3"}
 def summarize_trace(path):
     """Per-(stage kernel, precision) launch statistics of a rocprofv3 kernel trace
     (kernel_trace.csv) of a `--no-extra --tune-cache` run, over the most frequent launch
-    shape: lines `kernel precision mean_us median_us launches workgroups vgpr lds_bytes`
-    (the figures profiled_ms reads)."""
+    shape: lines `kernel precision b2b_mean_us mean_us median_us launches b2b_launches
+    workgroups vgpr lds_bytes`.  b2b: the launches that follow a launch of the same kernel
+    (sht_time_stage's back-to-back launches, which bench.py's kernel_ms times with HIP
+    events); mean / median over every launch (inside round trips too)."""
     import csv
     groups = {}
+    prev = None
     for kernel, mode, r in _stage_dispatches(list(csv.DictReader(open(path))), "Start_Timestamp"):
         wg = int(r["Grid_Size_X"]) // max(int(r["Workgroup_Size_X"]), 1)
         g = groups.setdefault((kernel, mode), {}).setdefault((wg, r["VGPR_Count"], r["LDS_Block_Size"]), [])
-        g.append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3)
-    lines = ["# kernel precision mean_us median_us launches workgroups vgpr lds_bytes"]
+        g.append(((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3, prev == (kernel, mode)))
+        prev = (kernel, mode)
+    lines = ["# kernel precision b2b_mean_us mean_us median_us launches b2b_launches workgroups vgpr lds_bytes"]
     for (kernel, mode), byshape in sorted(groups.items()):
         (wg, vgpr, lds), d = max(byshape.items(), key=lambda kv: len(kv[1]))
-        lines.append(f"{kernel} {mode} {np.mean(d):.2f} {np.median(d):.2f} {len(d)} {wg} {vgpr} {lds}")
+        allt = [t for t, _ in d]
+        b2b = [t for t, rep in d if rep] or allt
+        lines.append(f"{kernel} {mode} {np.mean(b2b):.2f} {np.mean(allt):.2f} {np.median(allt):.2f} {len(allt)} "
+                     f"{len(b2b)} {wg} {vgpr} {lds}")
     return "\n".join(lines)
 
 
@@ -387,7 +398,7 @@ def summarize_counters(paths):
         groups.setdefault((kernel, mode), {}).setdefault(shape, []).append(r["_c"])
     lines = [f"# copy calibration: bytes = FETCH_SIZE x 1024 x {scale.get('FETCH_SIZE', float('nan')):.4f}, "
              f"WRITE_SIZE x 1024 x {scale.get('WRITE_SIZE', float('nan')):.4f}",
-             "# kernel precision launches workgroups fetch_bytes write_bytes mfma_busy_pct mfma_gflop "
+             "# kernel precision fetch_bytes write_bytes launches workgroups mfma_busy_pct mfma_gflop "
              "lds_conflict_pct_of_active"]
     for (kernel, mode), byshape in sorted(groups.items()):
         (wg, vgpr, lds), ds = max(byshape.items(), key=lambda kv: len(kv[1]))
@@ -401,7 +412,7 @@ def summarize_counters(paths):
         flop = 512 * sum(mean(c) for c in ("SQ_INSTS_VALU_MFMA_MOPS_F16", "SQ_INSTS_VALU_MFMA_MOPS_BF16",
                                              "SQ_INSTS_VALU_MFMA_MOPS_F32")) * 1e-9
         conf = 100 * mean("SQ_LDS_BANK_CONFLICT") / mean("SQ_LDS_IDX_ACTIVE")
-        lines.append(f"{kernel} {mode} {len(ds)} {wg} {fetch:.0f} {write:.0f} {busy:.1f} {flop:.3f} {conf:.1f}")
+        lines.append(f"{kernel} {mode} {fetch:.0f} {write:.0f} {len(ds)} {wg} {busy:.1f} {flop:.3f} {conf:.1f}")
     return "\n".join(lines)
 
 
@@ -540,7 +551,7 @@ def main():
     dominant = max(KERNELS, key=lambda k: kms[k])
     roof = roofline(dominant, args.precision, kms[dominant])
     roof["traffic"] = measured_traffic(dominant, args.precision)
-    roof["traffic_source"] = "profiles/traffic_t255_k64.txt (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+    roof["traffic_source"] = "profiles/counters_t255_k64.txt (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
     roof["kernel"] = dominant
     roof["kernel_ms"] = kms[dominant]
     pm = profiled_ms(dominant, args.precision)

@@ -277,17 +277,21 @@ int launch_spectral_op(const DevPlan *P, int nharm, int op, const void *a, const
 // field[k][p] *= scale[p] for p < npoints, k < K (column stride ld)
 int launch_scale_points(const float *scale, void *field, long long npoints, long long ld, int K, void *stream);
 
-// Dense transform GEMM (split-K): launch shape and workspace of C = A B with A M x N, K columns.
+// Dense transform GEMM on MFMA (split-K): launch shape of C = A B with A M x N, K columns.
 struct DenseSplit {
-    int kb;  // columns of B per workgroup (8 or 16)
-    int rb;  // row blocks of 1024
-    int cg;  // column groups
-    int s;   // splits of N
-    int ns;  // columns of A per split
+    long long rt;  // row tiles of 16
+    long long s4;  // fragment groups of 16 contraction values
+    int cg;        // column groups of 64
+    int s;         // splits of the contraction (1: C written directly)
+    int s4_split;  // fragment groups per split
 };
 DenseSplit dense_split(long long M, long long N, int K);
-// C = A B (optionally rows times row_scale); work holds dense_split(M, N, K).s * K * M floats.
-int launch_dense_gemm(const float *A, long long M, long long N, const float *B, long long ldb, int K, float *C,
+// Position of A[r][n] (M x N) in the fragment order the GEMM reads (an array of
+// 16 ceil(M/16) x 16 ceil(N/16) floats, zero where r >= M or n >= N).
+long long dense_fragment_index(long long r, long long n, long long S4);
+// C = A B (optionally rows times row_scale), A in fragment order; work holds
+// dense_split(M, N, K).s * K * M floats when that split count exceeds 1.
+int launch_dense_gemm(const float *Af, long long M, long long N, const float *B, long long ldb, int K, float *C,
                       long long ldc, const float *row_scale, float *work, void *stream);
 int launch_uv_from_vordiv(const DevPlan *P, int nharm, const void *vor, const void *div, void *U,
                           void *V, long long ld, int K, float radius, void *stream);

@@ -370,8 +370,8 @@ struct sht_plan {
     // dense matrix transform (sht_plan_create_matrix): sht_synthesis / sht_analysis run as
     // one FP32 GEMM each with these column-major matrices (complex sides interleaved re, im)
     bool matrix = false;
-    float *mat_syn = nullptr;      // npoints x 2 nharm: columns (Re B, -Im B) of field = Re(B a)
-    float *mat_ana = nullptr;      // 2 nharm x npoints: rows (Re F, Im F) of a = F f
+    float *mat_syn = nullptr;      // npoints x 2 nharm (fragment order): columns (Re B, -Im B) of field = Re(B a)
+    float *mat_ana = nullptr;      // 2 nharm x npoints (fragment order): rows (Re F, Im F) of a = F f
     float *pt_coslat_inv = nullptr;  // 1/cos(lat) per grid point (unscale_coslat)
 };
 
@@ -1076,7 +1076,10 @@ static int build_matrices(sht_plan *pl) {
                                              std::to_string((long long)(bytes / 1048576)) + " MiB (limit 4 GiB)");
     std::vector<double> x, w;
     gauss_legendre(2 * n, x, w);
-    std::vector<float> B((size_t)np * 2 * nh, 0.f), F((size_t)2 * nh * np, 0.f), ci(np);
+    // both matrices in the GEMM's fragment order (sht::dense_fragment_index)
+    const long long S4s = (2 * nh + 15) / 16, S4a = (np + 15) / 16;
+    std::vector<float> B((size_t)((np + 15) / 16) * 16 * S4s * 16, 0.f), F((size_t)((2 * nh + 15) / 16) * 16 * S4a * 16, 0.f),
+        ci(np);
     std::vector<double> lam(nh);
     for (int jn = 0; jn < n; ++jn) {
         const double cosc = -x[jn], sinc = std::sqrt(std::max(0.0, 1.0 - x[jn] * x[jn]));
@@ -1095,10 +1098,10 @@ static int build_matrices(sht_plan *pl) {
                     for (int l = m; l < lmax; ++l) {
                         const long long lm = g.lm_off[m] + (l - m);
                         const double v = (hemi == 1 && ((l - m) & 1)) ? -lam[lm] : lam[lm];
-                        B[(size_t)(2 * lm) * np + p] = (float)(v * wm * c);
-                        B[(size_t)(2 * lm + 1) * np + p] = (float)(-v * wm * sn);
-                        F[(size_t)p * 2 * nh + 2 * lm] = (float)(dO * v * c);
-                        F[(size_t)p * 2 * nh + 2 * lm + 1] = (float)(-dO * v * sn);
+                        B[sht::dense_fragment_index(p, 2 * lm, S4s)] = (float)(v * wm * c);
+                        B[sht::dense_fragment_index(p, 2 * lm + 1, S4s)] = (float)(-v * wm * sn);
+                        F[sht::dense_fragment_index(2 * lm, p, S4a)] = (float)(dO * v * c);
+                        F[sht::dense_fragment_index(2 * lm + 1, p, S4a)] = (float)(-dO * v * sn);
                     }
                 }
             }
@@ -1115,8 +1118,9 @@ static int build_matrices(sht_plan *pl) {
     // workspace of the split-K GEMMs: the largest over both directions and every K
     long long work = 1;
     for (int K = 1; K <= pl->d.nlayers; ++K) {
-        work = std::max(work, (long long)sht::dense_split(np, 2 * nh, K).s * K * np);
-        work = std::max(work, (long long)sht::dense_split(2 * nh, np, K).s * K * 2 * nh);
+        const sht::DenseSplit ds = sht::dense_split(np, 2 * nh, K), da = sht::dense_split(2 * nh, np, K);
+        if (ds.s > 1) work = std::max(work, (long long)ds.s * K * np);
+        if (da.s > 1) work = std::max(work, (long long)da.s * K * 2 * nh);
     }
     pl->dense_bytes = ((size_t)work * 4 + 255) & ~(size_t)255;  // in each stream's scratch
     pl->device_bytes += (long long)bytes + np * 4 + work * 4;

@@ -143,85 +143,103 @@ int launch_scale_points(const float *scale, void *field, long long npoints, long
 }
 
 // ============================================================ dense transform GEMM
-// C (M x K, column stride ldc) = A (M x N, column-major, column stride M) B (N x K, column
-// stride ldb), then optionally C[r, k] *= row_scale[r].  The dense transform's products
-// have few columns (K layers) and a large A that must be read once, so this is a split-K
-// skinny GEMM bound by reading A: workgroup (row block, split, column group) reads rows
-// 1024 b .. + 1023 (thread t: rows t + 256 i, i < 4, so each load instruction reads 1 KiB
-// of a column) of the columns of its split, with the split's rows of B (KB columns) staged
-// in LDS and read as broadcasts; partial sums go to W[s][k][r], which a second kernel adds
-// up in a fixed order (no atomics, deterministic).
+// C (M x K, column stride ldc) = A (M x N) B (N x K, column stride ldb), then optionally
+// C[r, k] *= row_scale[r].  The dense transform's products have few columns (K layers) and
+// a large A that must be read once, so the GEMM runs on v_mfma_f32_16x16x4_f32 with A in
+// fragment order (dense_fragment_index: each wave's load of four K-steps is 1 KiB
+// contiguous) and is bound by reading A for K up to 32.  Workgroup = four waves, each one
+// 16-row tile, times a split of the contraction; the split's rows of B (all K columns, up
+// to 64 per column group) are staged in LDS 64 at a time (all of a thread's loads in flight
+// before its stores) and shared by the four row tiles, and each wave's A loads for the next
+// 64 rows are issued before the current ones' products.  With one split the result is stored directly; otherwise partial sums go to
+// W[s][k][r] and a second kernel adds them in a fixed order (no atomics, deterministic).
 
-constexpr int kDenseRows = 1024;  // rows per workgroup (4 per thread)
-constexpr int kDenseMaxNS = 256;  // columns of A per split at most (LDS for B)
+constexpr int kDenseChunk = 4;  // fragment groups (16 contraction values each) staged at once
+constexpr int kDenseCols = 64;  // columns of B per column group (four column tiles)
 
-// U columns of A per register set; two sets alternate, so the next set's loads are in
-// flight while the current set's products run (loads past the split re-read its last
-// column and are not used).
-template <int KB>
-__global__ __launch_bounds__(256) void dense_partial_kernel(const float *__restrict__ A_, long long M, long long N,
-                                                            const float *__restrict__ B_, long long ldb, int K,
-                                                            int ns, float *__restrict__ W_) {
-    constexpr int U = KB == 8 ? 8 : 4;
-    __shared__ float Bs[kDenseMaxNS][KB];
-    const int tid = threadIdx.x;
-    const long long r0 = (long long)blockIdx.x * kDenseRows + tid;
-    const int sp = blockIdx.y, k0 = blockIdx.z * KB;
-    const long long n0 = (long long)sp * ns;
-    const int nn = (int)min((long long)ns, N - n0);  // >= 1: the host makes every split non-empty
-    const SHT_GLOBAL float *B = as_global(B_);
-    for (int i = tid; i < nn * KB; i += 256) {
-        const int q = i / KB, k = i - q * KB;
-        Bs[q][k] = k0 + k < K ? B[(n0 + q) + (long long)(k0 + k) * ldb] : 0.f;
-    }
-    int rr[4];  // rows past M re-read row M - 1 (unconditional loads); their sums are dropped
+template <int NCT>
+__global__ __launch_bounds__(256) void dense_mfma_kernel(const float4 *__restrict__ Af_, long long RT, long long S4,
+                                                         long long M, long long N, const float *__restrict__ B_,
+                                                         long long ldb, int K, int s4_split, float *__restrict__ out_,
+                                                         long long ldo, int direct,
+                                                         const float *__restrict__ row_scale) {
+    constexpr int BST = 16 * NCT + 16;  // LDS row stride: the four lane groups' rows on distinct banks
+    __shared__ float Bs[kDenseChunk * 16][BST];
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int row = lane & 15, kg = lane >> 4;
+    const long long rt = (long long)blockIdx.x * 4 + w;  // idle waves (rt >= RT) still stage and synchronise
+    const int sp = blockIdx.y;
+    const int k0 = blockIdx.z * kDenseCols, Kg = min(kDenseCols, K - k0);
+    const long long s4a = (long long)sp * s4_split, s4b = min(S4, s4a + s4_split);
+    const SHT_GLOBAL float *B = as_global(B_) + (long long)k0 * ldb;
+    const SHT_GLOBAL fv4 *Aw = reinterpret_cast<const SHT_GLOBAL fv4 *>(as_global(Af_)) +
+                               (min(rt, RT - 1) * S4) * 64 + lane;
+    f4 acc[NCT];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) rr[i] = (int)min(r0 + 256 * i, M - 1);
-    float acc[4][KB];
+    for (int ct = 0; ct < NCT; ++ct) acc[ct] = f4{0.f, 0.f, 0.f, 0.f};
+    // B staging: thread t takes rows q = (t + 256 u) % 128 of column j = (t + 256 u) / 128,
+    // all loads of a chunk in flight before its LDS stores
+    constexpr int CR = kDenseChunk * 16;                // rows staged per chunk
+    constexpr int PER = CR * 16 * NCT / 256;            // B values per thread and chunk
+    // this wave's A fragments: the next chunk's loads are issued before the current
+    // chunk's products (two register sets)
+    fv4 a0[kDenseChunk], a1[kDenseChunk];
+    auto load_a = [&](fv4 (&a)[kDenseChunk], long long c0) {
+        const long long nch = min((long long)kDenseChunk, s4b - c0);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int k = 0; k < KB; ++k) acc[i][k] = 0.f;
-    const SHT_GLOBAL float *A = as_global(A_) + n0 * M;
-    auto fetch = [&](float (&a)[U][4], int q) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const SHT_GLOBAL float *col = A + (long long)min(q + u, nn - 1) * M;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[u][i] = col[rr[i]];
-        }
+        for (int c = 0; c < kDenseChunk; ++c) a[c] = Aw[(c0 + min((long long)c, nch - 1)) * 64];
     };
-    auto products = [&](const float (&a)[U][4], int q) {
+    auto chunk = [&](const fv4 (&a)[kDenseChunk], long long c0) {
+        const int nch = (int)min((long long)kDenseChunk, s4b - c0);
+        float bv[PER];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (q + u >= nn) break;  // uniform
+        for (int u = 0; u < PER; ++u) {
+            const int i = tid + 256 * u, q = i % CR, j = i / CR;
+            const long long n = 16 * c0 + q;
+            bv[u] = (q < 16 * nch && n < N && j < Kg) ? B[n + (long long)j * ldb] : 0.f;
+        }
+        __syncthreads();  // the previous chunk's B is no longer read
 #pragma unroll
-            for (int k = 0; k < KB; ++k) {
-                const float b = Bs[q + u][k];
+        for (int u = 0; u < PER; ++u) {
+            const int i = tid + 256 * u;
+            Bs[i % CR][i / CR] = bv[u];
+        }
+        __syncthreads();
 #pragma unroll
-                for (int i = 0; i < 4; ++i) acc[i][k] = fmaf(a[u][i], b, acc[i][k]);
+        for (int c = 0; c < kDenseChunk; ++c) {
+            if (c >= nch) break;  // uniform
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int q = 16 * c + 4 * v + kg;
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct) acc[ct] = mfma_f32(a[c][v], Bs[q][ct * 16 + row], acc[ct]);
             }
         }
     };
-    float a0[U][4], a1[U][4];
-    fetch(a0, 0);
-    __syncthreads();  // B staged
-    int q = 0;
-    for (; q + U < nn; q += 2 * U) {  // straight-line body: each set's wait covers its own loads
-        fetch(a1, q + U);
-        products(a0, q);
-        fetch(a0, q + 2 * U);
-        products(a1, q + U);
+    long long c0 = s4a;
+    if (c0 < s4b) load_a(a0, c0);
+    for (; c0 < s4b; c0 += 2 * kDenseChunk) {  // uniform; straight-line body, each set waits for its own loads
+        if (c0 + kDenseChunk < s4b) load_a(a1, c0 + kDenseChunk);
+        chunk(a0, c0);
+        if (c0 + kDenseChunk >= s4b) break;
+        if (c0 + 2 * kDenseChunk < s4b) load_a(a0, c0 + 2 * kDenseChunk);
+        chunk(a1, c0 + kDenseChunk);
     }
-    if (q < nn) products(a0, q);
-    SHT_GLOBAL float *W = as_global(W_);
+    if (rt >= RT) return;
+    SHT_GLOBAL float *out = as_global(out_);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const long long r = r0 + 256 * i;
-        if (r >= M) continue;
+    for (int ct = 0; ct < NCT; ++ct) {
+        const int k = ct * 16 + row;
+        if (k >= Kg) continue;
 #pragma unroll
-        for (int k = 0; k < KB; ++k)
-            if (k0 + k < K) W[((long long)sp * K + k0 + k) * M + r] = acc[i][k];
+        for (int i = 0; i < 4; ++i) {
+            const long long r = rt * 16 + 4 * kg + i;
+            if (r >= M) continue;
+            if (direct)
+                out[r + (long long)(k0 + k) * ldo] = acc[ct][i] * (row_scale ? as_global(row_scale)[r] : 1.f);
+            else
+                out[((long long)sp * K + k0 + k) * M + r] = acc[ct][i];
+        }
     }
 }
 
@@ -253,31 +271,43 @@ __global__ __launch_bounds__(256) void dense_reduce_kernel(const float *__restri
 
 DenseSplit dense_split(long long M, long long N, int K) {
     DenseSplit d;
-    d.kb = K <= 8 ? 8 : 16;
-    d.rb = (int)((M + kDenseRows - 1) / kDenseRows);
-    d.cg = (K + d.kb - 1) / d.kb;
-    long long s = (512 + (long long)d.rb * d.cg - 1) / ((long long)d.rb * d.cg);  // about 512 workgroups
-    s = std::min(s, (N + 31) / 32);                                               // at least 32 columns each
-    s = std::max(s, (N + kDenseMaxNS - 1) / kDenseMaxNS);                         // at most kDenseMaxNS
-    s = std::max(s, 1LL);
-    d.ns = (int)((N + s - 1) / s);
-    d.s = (int)((N + d.ns - 1) / d.ns);  // every split non-empty
+    d.rt = (M + 15) / 16;
+    d.s4 = (N + 15) / 16;
+    d.cg = (K + kDenseCols - 1) / kDenseCols;
+    const long long wgs = (d.rt + 3) / 4 * d.cg;
+    long long s = (2048 + wgs - 1) / wgs;  // about 2048 workgroups
+    s = std::max(1LL, std::min(s, d.s4));
+    d.s4_split = (int)((d.s4 + s - 1) / s);
+    d.s = (int)((d.s4 + d.s4_split - 1) / d.s4_split);  // every split non-empty
     return d;
 }
 
-int launch_dense_gemm(const float *A, long long M, long long N, const float *B, long long ldb, int K, float *C,
+long long dense_fragment_index(long long r, long long n, long long S4) {
+    const long long rt = r / 16, s4 = n / 16, q = n % 16;
+    const long long lane = (r % 16) + 16 * (q % 4), v = q / 4;
+    return ((rt * S4 + s4) * 64 + lane) * 4 + v;
+}
+
+int launch_dense_gemm(const float *Af, long long M, long long N, const float *B, long long ldb, int K, float *C,
                       long long ldc, const float *row_scale, float *work, void *stream) {
     if (M < 1 || N < 1 || K < 1 || K > 65535 || ldb < N || ldc < M) return (int)hipErrorInvalidValue;
     const DenseSplit d = dense_split(M, N, K);
-    if (d.s > 65535 || d.cg > 65535) return (int)hipErrorInvalidValue;
+    if ((d.rt + 3) / 4 > 0x7fffffffLL || d.s > 65535 || d.cg > 65535) return (int)hipErrorInvalidValue;
     const hipStream_t st = as_stream(stream);
-    const dim3 grid(d.rb, d.s, d.cg);
-    if (d.kb == 8)
-        dense_partial_kernel<8><<<grid, 256, 0, st>>>(A, M, N, B, ldb, K, d.ns, work);
-    else
-        dense_partial_kernel<16><<<grid, 256, 0, st>>>(A, M, N, B, ldb, K, d.ns, work);
+    const dim3 grid((unsigned)((d.rt + 3) / 4), d.s, d.cg);
+    const bool direct = d.s == 1;
+    float *out = direct ? C : work;
+    const long long ldo = direct ? ldc : M;
+    const float4 *A4 = reinterpret_cast<const float4 *>(Af);
+    const int nct = std::min(4, (std::min(K, kDenseCols) + 15) / 16);
+    switch (nct) {
+        case 1: dense_mfma_kernel<1><<<grid, 256, 0, st>>>(A4, d.rt, d.s4, M, N, B, ldb, K, d.s4_split, out, ldo, direct, row_scale); break;
+        case 2: dense_mfma_kernel<2><<<grid, 256, 0, st>>>(A4, d.rt, d.s4, M, N, B, ldb, K, d.s4_split, out, ldo, direct, row_scale); break;
+        case 3: dense_mfma_kernel<3><<<grid, 256, 0, st>>>(A4, d.rt, d.s4, M, N, B, ldb, K, d.s4_split, out, ldo, direct, row_scale); break;
+        default: dense_mfma_kernel<4><<<grid, 256, 0, st>>>(A4, d.rt, d.s4, M, N, B, ldb, K, d.s4_split, out, ldo, direct, row_scale); break;
+    }
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return (int)e;
+    if (e != hipSuccess || direct) return (int)e;
     dense_reduce_kernel<<<dim3((unsigned)((M + kReduceRows - 1) / kReduceRows), K), 256, 0, st>>>(work, M, K, d.s,
                                                                                                 row_scale, C, ldc);
     return (int)hipGetLastError();

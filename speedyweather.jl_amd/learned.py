@@ -129,11 +129,12 @@ def forecast_score(net, nx, ny, seeds, spinup_hours=48.0, window_hours=1.0, memb
 
 
 # How the correction couples into the model (corrector keyword arguments): all of its
-# divergence, or only the divergence's departure from the column mean (the column mean
-# drives the surface-pressure tendency through continuity), with or without its own ln ps
-# tendency.
+# divergence, none of it, or only the divergence's departure from the column mean (the column
+# mean drives the surface-pressure tendency through continuity), with or without its own
+# ln ps tendency.
 CORRECTOR_VARIANTS = {
     "uvT": {"fields": ("vor", "div", "temp")},
+    "vorT": {"fields": ("vor", "temp")},
     "uvT_baroclinic_div": {"fields": ("vor", "div", "temp"), "baroclinic_div": True},
     "uvTq_baroclinic_div": {"fields": ("vor", "div", "temp", "lnps"), "baroclinic_div": True},
 }
