@@ -139,7 +139,8 @@ def bench_c2(sw, torch, steps, warmup):
 def bench_dense(sw, torch, steps, warmup):
     """Round trips of the dense matrix transform (one FP32 GEMM per direction) against the
     regular transform at small truncations, where the reference uses the dense one on GPUs:
-    T31 (nlat_half 24, K=8) and T63 (nlat_half 48, K=32), FP32."""
+    T31 (nlat_half 24, K=8) and T63 (nlat_half 48, K=32), FP32; the regular transform also
+    with its calls replayed from cached graphs (gpu_graphs)."""
     out = {}
     for name, n, lmax, mmax, K in (("T31_K8", 24, 33, 32, 8), ("T63_K32", 48, 65, 64, 32)):
         sp, grid = sw.Spectrum(lmax, mmax), sw.OctahedralGaussianGrid(n)
@@ -147,8 +148,11 @@ def bench_dense(sw, torch, steps, warmup):
         f = sw.Field(torch.empty((K, grid.npoints), device="cuda"), grid)
         b = sw.LowerTriangularArray(torch.empty_like(c.data), sp)
         row = {}
-        for tag, cls in (("dense_ms", sw.MatrixSpectralTransform), ("sht_ms", sw.SpectralTransform)):
+        for tag, cls, graphs in (("dense_ms", sw.MatrixSpectralTransform, False),
+                                 ("sht_ms", sw.SpectralTransform, False), ("sht_graphs_ms", sw.SpectralTransform, True)):
             S = cls(sp, grid, nlayers=K, precision="fp32")
+            if graphs:
+                S.set_graphs(True)
 
             def step():
                 sw.transform_(f, c, S)

@@ -276,6 +276,21 @@ int sht_tune_cache_load(const char *path, int32_t *nloaded);
  * to be called concurrently with other calls on the plan from another host thread. */
 int sht_plan_set_tuning(sht_plan *plan, const int32_t *in);
 
+/* Graph replay of single calls (the reference caches a captured graph of its Fourier stage
+ * per transform and buffer on CUDA: SpeedyTransforms/ext/gpu_graphs_common.jl:23,121,293-331,
+ * MAX_GRAPHS = 64, capture failure -> direct loop).  With graphs on, sht_synthesis /
+ * sht_analysis and the calls built on them replay an explicitly built graph of their two
+ * stage kernels, one per distinct call (direction, buffers, strides, K, flags, scratch,
+ * tuning), built on its first use while fewer than 64 per direction are cached; a call whose
+ * graph fails to build or instantiate, calls beyond the cap, calls on a capturing stream and
+ * calls while profiling launch directly, and so do a dense matrix plan's GEMM calls.  Results
+ * are identical either way.  Off by default.
+ * SHT_GRAPHS_FORCE_FAILURE (tests): treat every instantiation as failed. */
+#define SHT_GRAPHS_FORCE_FAILURE 1
+int sht_plan_set_graphs(sht_plan *plan, int32_t enable, int32_t flags);
+/* graphs cached on the plan (both directions) and graphs that failed */
+int sht_plan_graph_count(sht_plan *plan, int32_t *cached, int32_t *failed);
+
 /* Per-kernel timing (no reference counterpart; the reference profiled ad hoc with
  * CUDA.@profile / BenchmarkTools, docs/dev/2026-08/legendre-gpu-optimization.md).
  * Between begin and end, the first max_calls launches of every kernel are bracketed by

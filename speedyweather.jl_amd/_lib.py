@@ -30,7 +30,7 @@ EXPORTS = (
     "sht_plan_autotune", "sht_plan_tuning", "sht_plan_set_tuning", "sht_spec2grid_pullback",
     "sht_grid2spec_pullback", "sht_divergence", "sht_laplace", "sht_gradient", "sht_plan_create_matrix",
     "sht_synthesis_laplace", "sht_analysis_laplace", "sht_time_stage", "sht_tune_cache_save",
-    "sht_tune_cache_load", "sht_transform_static",
+    "sht_tune_cache_load", "sht_transform_static", "sht_plan_set_graphs", "sht_plan_graph_count",
 )
 
 
@@ -83,6 +83,8 @@ def lib() -> C.CDLL:
         L.sht_synthesis.argtypes = [vp, vp, i64, vp, i64, i32, i32, vp]
         L.sht_analysis.argtypes = [vp, vp, i64, vp, i64, i32, vp]
         L.sht_transform_static.argtypes = [vp, i32, vp, i64, vp, i64, i32, i32, vp]
+        L.sht_plan_set_graphs.argtypes = [vp, i32, i32]
+        L.sht_plan_graph_count.argtypes = [vp, vp, vp]
         L.sht_spec2grid_pullback.argtypes = [vp, vp, i64, vp, i64, i32, i32, i32, vp]
         L.sht_grid2spec_pullback.argtypes = [vp, vp, i64, vp, i64, i32, i32, vp]
         L.sht_divergence.argtypes = [vp, vp, vp, vp, i64, i32, i32, i32, i32, C.c_double, vp]

@@ -373,6 +373,15 @@ struct sht_plan {
     float *mat_syn = nullptr;      // npoints x 2 nharm (fragment order): columns (Re B, -Im B) of field = Re(B a)
     float *mat_ana = nullptr;      // 2 nharm x npoints (fragment order): rows (Re F, Im F) of a = F f
     float *pt_coslat_inv = nullptr;  // 1/cos(lat) per grid point (unscale_coslat)
+    // Graph replay of single calls (sht_plan_set_graphs; the reference's per-transform graph
+    // cache, SpeedyTransforms/ext/gpu_graphs_common.jl:23,121,293-331): an explicitly built,
+    // instantiated graph of a call's two stage kernels per distinct call (direction, buffers,
+    // strides, K, flags, scratch, tuning), at most kMaxGraphs per direction; a null entry
+    // marks a call whose graph failed, which then launches directly.
+    bool graphs = false, graph_force_failure = false;
+    std::mutex graph_mutex;
+    std::map<std::string, hipGraphExec_t> graph_cache[2];
+    int graph_failed = 0;
 };
 
 namespace {
@@ -416,6 +425,9 @@ struct DeviceGuard {
 
 void free_plan(sht_plan *p) {
     if (!p) return;
+    for (auto &gc : p->graph_cache)
+        for (auto &kv : gc)
+            if (kv.second) (void)hipGraphExecDestroy(kv.second);
     for (hipEvent_t e : p->prof.ev) (void)hipEventDestroy(e);
     for (void *a : p->allocs) (void)hipFree(a);
     for (auto &sb : p->scratch) (void)hipFree(sb.second);
@@ -1208,38 +1220,121 @@ struct LapOp {
 
 // Launches one stage of a single call (c: its arguments, K columns) on the plan's chosen
 // launch orders, bracketed by profiling events when profiling is on.
-static hipError_t launch_plan_stage(sht_plan *plan, int stage, const CallArgs &c, void *stream) {
+// Kernel, grid, dynamic LDS and tile list of one stage of a single call on the plan's chosen
+// launch orders.
+struct StageLaunch {
+    const void *func;
+    dim3 grid;
+    unsigned smem;
+    const Tile *tiles;
+};
+
+static StageLaunch plan_stage_launch(const sht_plan *plan, int stage, const CallArgs &c) {
     const int K = c.K;
     const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
     const int gx = plan->leg_xcd;
-    static const int kernel_id[4] = {SHT_KERNEL_LEGENDRE_SYNTHESIS, SHT_KERNEL_FFT_SYNTHESIS, SHT_KERNEL_FFT_ANALYSIS,
-                                     SHT_KERNEL_LEGENDRE_ANALYSIS};
-    ProfScope ps(plan, kernel_id[stage], stream);
     switch (stage) {
         case sht::kLegendreSynthesis:
-            return launch_stage(sht::legendre_kernel(stage, plan->d.precision, false), dim3(plan->nsyn[gx][cgs - 1]),
-                                0, plan->dP, c, nullptr, plan->syn_tiles[gx][cgs - 1], stream);
+            return {sht::legendre_kernel(stage, plan->d.precision, false), dim3(plan->nsyn[gx][cgs - 1]), 0,
+                    plan->syn_tiles[gx][cgs - 1]};
         case sht::kFFTSynthesis:
         case sht::kFFTAnalysis: {
             if (plan->fourier != sht::kFourierRadix) {  // ring DFT on MFMA: blocks (ring rank, column group)
                 const bool f16 = plan->fourier == sht::kFourierDft16;
                 const int cg = plan->dft_cg;
                 const bool syn = stage == sht::kFFTSynthesis;
-                return launch_stage(sht::dft_kernel(stage, f16, cg, false, syn && c.accumulate != 0),
-                                    dim3(plan->g.n * ((K + cg - 1) / cg)), plan->dft_smem[syn][f16][cg == 8],
-                                    plan->dP, c, nullptr, nullptr, stream);
+                return {sht::dft_kernel(stage, f16, cg, false, syn && c.accumulate != 0),
+                        dim3(plan->g.n * ((K + cg - 1) / cg)), (unsigned)plan->dft_smem[syn][f16][cg == 8], nullptr};
             }
             if (stage == sht::kFFTSynthesis)
-                return launch_stage(sht::fft_kernel(stage, false, c.accumulate != 0),
-                                    dim3(fft_blocks(plan->g.n, K, c.paired != 0)), plan->P.fft_smem, plan->dP, c,
-                                    nullptr, nullptr, stream);
-            return launch_stage(sht::fft_kernel(stage, false), dim3(fft_blocks(plan->g.n, K, false)), plan->P.fft_smem,
-                                plan->dP, c, nullptr, nullptr, stream);
+                return {sht::fft_kernel(stage, false, c.accumulate != 0), dim3(fft_blocks(plan->g.n, K, c.paired != 0)),
+                        (unsigned)plan->P.fft_smem, nullptr};
+            return {sht::fft_kernel(stage, false), dim3(fft_blocks(plan->g.n, K, false)), (unsigned)plan->P.fft_smem,
+                    nullptr};
         }
         default:
-            return launch_stage(sht::legendre_kernel(stage, plan->d.precision, false), dim3(plan->nana[gx][cgs - 1]),
-                                0, plan->dP, c, nullptr, plan->ana_tiles[gx][cgs - 1], stream);
+            return {sht::legendre_kernel(stage, plan->d.precision, false), dim3(plan->nana[gx][cgs - 1]), 0,
+                    plan->ana_tiles[gx][cgs - 1]};
     }
+}
+
+// Launches one stage of a single call, bracketed by profiling events when profiling is on.
+static hipError_t launch_plan_stage(sht_plan *plan, int stage, const CallArgs &c, void *stream) {
+    static const int kernel_id[4] = {SHT_KERNEL_LEGENDRE_SYNTHESIS, SHT_KERNEL_FFT_SYNTHESIS, SHT_KERNEL_FFT_ANALYSIS,
+                                     SHT_KERNEL_LEGENDRE_ANALYSIS};
+    ProfScope ps(plan, kernel_id[stage], stream);
+    const StageLaunch l = plan_stage_launch(plan, stage, c);
+    return launch_stage(l.func, l.grid, l.smem, plan->dP, c, nullptr, l.tiles, stream);
+}
+
+constexpr int kMaxGraphs = 64;  // cached graphs per direction and plan
+
+// Runs the two stages of a call (s0 then s1) from the plan's graph cache when graphs are on:
+// replays the call's graph, or builds and instantiates it (explicit kernel nodes, no stream
+// capture) while fewer than kMaxGraphs are cached.  Returns false when the call is to be
+// launched directly (graphs off, profiling on, a capturing stream, a full cache, or a
+// failed graph); e = the launch status otherwise.
+static bool launch_from_graph(sht_plan *plan, int dir, int s0, int s1, const CallArgs &c, void *stream,
+                              hipError_t &e) {
+    if (!plan->graphs || plan->prof.on) return false;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(reinterpret_cast<hipStream_t>(stream), &cs) != hipSuccess ||
+        cs != hipStreamCaptureStatusNone)
+        return false;
+    // key: the call's fields one by one (CallArgs has tail padding) and the tuning
+    std::string key;
+    auto put = [&key](const auto &v) { key.append(reinterpret_cast<const char *>(&v), sizeof(v)); };
+    put(c.K), put(c.unscale), put(c.paired), put(c.in), put(c.out), put(c.ld_in), put(c.ld_out), put(c.fbuf);
+    put(c.adj), put(c.accumulate), put(c.lap), put(c.lap_sign), put(c.lap_scale);
+    put(plan->pol), put(plan->leg_xcd), put(plan->fft_pair), put(plan->fourier), put(plan->dft_cg);
+    hipGraphExec_t exec = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(plan->graph_mutex);
+        auto &cache = plan->graph_cache[dir];
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            if (!it->second) return false;  // its graph failed: direct launches
+            exec = it->second;
+        } else {
+            if ((int)cache.size() >= kMaxGraphs) return false;
+            hipGraph_t graph = nullptr;
+            bool ok = hipGraphCreate(&graph, 0) == hipSuccess;
+            hipGraphNode_t prev = nullptr;
+            for (int st : {s0, s1}) {
+                if (!ok) break;
+                const StageLaunch l = plan_stage_launch(plan, st, c);
+                const DevPlan *a0 = plan->dP;
+                CallArgs a1 = c;
+                const CallArgs *a2 = nullptr;
+                const Tile *a3 = l.tiles;
+                void *args[] = {&a0, &a1, &a2, &a3};
+                hipKernelNodeParams kp{};
+                kp.func = const_cast<void *>(l.func);
+                kp.gridDim = l.grid;
+                kp.blockDim = dim3(256);
+                kp.sharedMemBytes = l.smem;
+                kp.kernelParams = args;
+                kp.extra = nullptr;
+                hipGraphNode_t node;
+                ok = hipGraphAddKernelNode(&node, graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp) == hipSuccess;
+                prev = node;
+            }
+            if (ok && !plan->graph_force_failure)
+                ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+            else
+                ok = false;
+            if (graph) (void)hipGraphDestroy(graph);
+            if (!ok) {
+                (void)hipGetLastError();
+                exec = nullptr;
+                ++plan->graph_failed;
+            }
+            cache[key] = exec;
+            if (!exec) return false;
+        }
+    }
+    e = hipGraphLaunch(exec, reinterpret_cast<hipStream_t>(stream));
+    return true;
 }
 
 // Scratch of a call: the caller's (sized for the plan), else the stream's own.
@@ -1270,7 +1365,12 @@ static int run_synthesis(sht_plan *plan, const void *coeffs, int64_t ld_spec, vo
     }
     const CallArgs c{0, K, unscale_coslat ? 1 : 0, plan->fft_pair, coeffs, field, ld_spec, ld_grid, ws,
                      adj, accumulate ? 1 : 0, op.lap, op.sign, op.scale};
-    hipError_t e = launch_plan_stage(plan, sht::kLegendreSynthesis, c, stream);
+    hipError_t e;
+    if (launch_from_graph(plan, 0, sht::kLegendreSynthesis, sht::kFFTSynthesis, c, stream, e)) {
+        if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("synthesis graph launch: ") + hipGetErrorString(e));
+        return SHT_OK;
+    }
+    e = launch_plan_stage(plan, sht::kLegendreSynthesis, c, stream);
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("legendre synthesis launch: ") + hipGetErrorString(e));
     e = launch_plan_stage(plan, sht::kFFTSynthesis, c, stream);
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("fft synthesis launch: ") + hipGetErrorString(e));
@@ -1299,7 +1399,12 @@ static int run_analysis(sht_plan *plan, const void *field, int64_t ld_grid, void
     }
     const CallArgs c{0,  K,   unscale_coslat ? 1 : 0, 0,      field,   coeffs, ld_grid, ld_spec,
                      ws, adj, accumulate ? 1 : 0,     op.lap, op.sign, op.scale};
-    hipError_t e = launch_plan_stage(plan, sht::kFFTAnalysis, c, stream);
+    hipError_t e;
+    if (launch_from_graph(plan, 1, sht::kFFTAnalysis, sht::kLegendreAnalysis, c, stream, e)) {
+        if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("analysis graph launch: ") + hipGetErrorString(e));
+        return SHT_OK;
+    }
+    e = launch_plan_stage(plan, sht::kFFTAnalysis, c, stream);
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("fft analysis launch: ") + hipGetErrorString(e));
     e = launch_plan_stage(plan, sht::kLegendreAnalysis, c, stream);
     if (e != hipSuccess) return fail(SHT_ERR_HIP, std::string("legendre analysis launch: ") + hipGetErrorString(e));
@@ -1732,6 +1837,25 @@ int sht_plan_autotune(sht_plan *plan, int32_t K, void *stream) {
     plan->tuned = 1;
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     g_tune_cache[key] = best;
+    return SHT_OK;
+}
+
+int sht_plan_set_graphs(sht_plan *plan, int32_t enable, int32_t flags) {
+    if (!plan || enable < 0 || enable > 1 || (flags & ~SHT_GRAPHS_FORCE_FAILURE)) return fail(SHT_ERR_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lock(plan->graph_mutex);
+    plan->graphs = enable != 0;
+    plan->graph_force_failure = (flags & SHT_GRAPHS_FORCE_FAILURE) != 0;
+    return SHT_OK;
+}
+
+int sht_plan_graph_count(sht_plan *plan, int32_t *cached, int32_t *failed) {
+    if (!plan) return fail(SHT_ERR_ARG, "null plan");
+    std::lock_guard<std::mutex> lock(plan->graph_mutex);
+    int n = 0;
+    for (auto &gc : plan->graph_cache)
+        for (auto &kv : gc) n += kv.second ? 1 : 0;
+    if (cached) *cached = n;
+    if (failed) *failed = plan->graph_failed;
     return SHT_OK;
 }
 

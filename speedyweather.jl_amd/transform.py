@@ -28,7 +28,7 @@ class SpectralTransform:
     three MFMA products, FP32 accumulation) or "bf16x3" (BF16 hi/lo split)."""
 
     def __init__(self, spectrum: Spectrum, grid, nlayers: int = 1, LegendreShortcut: str = "linear",
-                 precision: str = "fp32", device: int | None = None):
+                 precision: str = "fp32", device: int | None = None, gpu_graphs: bool = False):
         if grid.kind not in _lib.GRIDS:
             raise NotImplementedError(f"{type(grid).__name__} is not supported by the MI355X engine")
         self.spectrum, self.grid = spectrum, grid
@@ -43,9 +43,24 @@ class SpectralTransform:
                                   _lib.PRECISIONS[precision], self.device)
         self._plan = C.c_void_p()
         self._create()
+        if gpu_graphs:
+            self.set_graphs(True)
 
     def _create(self):
         check(lib().sht_plan_create(C.byref(self._desc), C.byref(self._plan)), "sht_plan_create")
+
+    def set_graphs(self, enable: bool = True, force_failure: bool = False):
+        """Graph replay of repeated calls (the reference's gpu_graphs, SpeedyTransforms/ext/
+        gpu_graphs_common.jl:293-331): each distinct call's two stage kernels replay from a cached
+        graph, at most 64 per direction (sht_plan_set_graphs)."""
+        check(lib().sht_plan_set_graphs(self._plan, int(bool(enable)), 1 if force_failure else 0),
+              "sht_plan_set_graphs")
+
+    def graph_count(self) -> tuple:
+        """(graphs cached, graphs that failed) (sht_plan_graph_count)."""
+        n, f = C.c_int32(), C.c_int32()
+        check(lib().sht_plan_graph_count(self._plan, C.byref(n), C.byref(f)), "sht_plan_graph_count")
+        return n.value, f.value
 
     @classmethod
     def from_grid(cls, grid, truncation: int = 0, dealiasing: float = 2.0, one_more_degree: bool = False,
