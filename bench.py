@@ -324,6 +324,7 @@ def profiled_ms(kernel, precision):
 
 
 MODE_NAMES = {"0": "fp32", "1": "fp16x3", "2": "bf16x3"}
+XCDS = 8  # MI355X
 
 
 def summarize_trace(path):
@@ -376,7 +377,9 @@ def summarize_counters(paths):
     (counter_collection.csv of `--no-extra --tune-cache --copy-calibration` runs), over the most
     frequent launch shape, with the derived figures: HBM bytes (FETCH_SIZE and WRITE_SIZE in
     KiB, each scaled by what it reads for the largest plain device copy of the run, whose bytes
-    are known), MFMA busy share (SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x 1024 SIMDs)),
+    are known), MFMA busy share (SQ_VALU_MFMA_BUSY_CYCLES, summed over the 1024 SIMDs, over
+    1024 x the kernel's cycles; GRBM_GUI_ACTIVE sums the 8 XCDs' cycles: it reads 22.7 to 23.4
+    per ns of kernel time on MI355X, 8 x 2.4 GHz plus the dispatch),
     MFMA flops (SQ_INSTS_VALU_MFMA_MOPS_* x 512) and the share of LDS-active cycles stalled by
     bank conflicts (SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE)."""
     import csv
@@ -412,7 +415,7 @@ def summarize_counters(paths):
             return float(np.mean(v)) if v else float("nan")
         fetch = mean("FETCH_SIZE") * 1024 * scale.get("FETCH_SIZE", float("nan"))
         write = mean("WRITE_SIZE") * 1024 * scale.get("WRITE_SIZE", float("nan"))
-        busy = 100 * mean("SQ_VALU_MFMA_BUSY_CYCLES") / (mean("GRBM_GUI_ACTIVE") * 1024)
+        busy = 100 * mean("SQ_VALU_MFMA_BUSY_CYCLES") / (mean("GRBM_GUI_ACTIVE") / XCDS * 1024)
         flop = 512 * sum(mean(c) for c in ("SQ_INSTS_VALU_MFMA_MOPS_F16", "SQ_INSTS_VALU_MFMA_MOPS_BF16",
                                              "SQ_INSTS_VALU_MFMA_MOPS_F32")) * 1e-9
         conf = 100 * mean("SQ_LDS_BANK_CONFLICT") / mean("SQ_LDS_IDX_ACTIVE")
