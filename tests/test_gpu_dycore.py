@@ -88,7 +88,10 @@ def test_truncation_tendency_learned(mods):
     t0 = {k: v.clone() for k, v in tend.items()}
     ln.corrector(net, nx, ny, baroclinic_div=True)(m, tend, g)
     dd = (tend["div"] - t0["div"]).view(2, 8, -1)
-    assert float(dd.mean(dim=1).abs().max()) <= 1e-6 * float(dd.abs().max())
+    # FP32: the difference carries the rounding of the sums t0 + correction (half an ulp of
+    # the tendency) besides the round-off of the correction's own mean removal
+    tol = 1e-6 * (float(t0["div"].abs().max()) + float(dd.abs().max()))
+    assert float(dd.mean(dim=1).abs().max()) <= tol
     assert float(dd.abs().max()) > 0 and torch.equal(tend["lnps"], t0["lnps"])
     m.close()
     # a corrected coarse run stays bounded
