@@ -38,8 +38,12 @@ against a brute-force DFT; the whole transform against Y00 -> constant grid, sin
 harmonic round trips, the FP64 grid->spec->grid round trip at cubic dealiasing, the
 vector path against solid-body rotation, the pullbacks against the dot-product identity
 with synthesis and analysis (they are their exact adjoints), the gradient operators against
-finite differences in latitude of the fields they act on, evaluated at arbitrary latitudes. Parity unpinned: the Condon-Shortley sign of
-odd-m polynomials inside AssociatedLegendrePolynomials.jl (no reference test fixes it).
+finite differences in latitude of the fields they act on, evaluated at arbitrary latitudes.
+The Condon-Shortley phase: the reference's documentation states the real-field symmetry
+a_{l,-m} = (-1)^m conj(a_{l,m}) (docs/src/spectral_transform.md:66-69), which holds for
+the phase-carrying convention that scipy's sph_harm_y also uses (the pin above); no
+reference test fixes the sign of odd-m polynomials numerically, and round trips do not
+depend on it.
 """
 from __future__ import annotations
 
