@@ -243,9 +243,13 @@ def bench_c4(sw, torch, steps, warmup):
             sw.transform_(mid, c, S)
             sw.transform_(out, mid, S)
     dt_seq = time_loop(torch, sequential, max(steps // 4, 5), max(warmup // 4, 2))
+    # compulsory bytes of every round trip (spectral in and out, grid out and in, Λ both ways)
+    b_rt = sum(2 * S.spectrum.nharm * c.nlayers * 8 + 2 * S.grid.npoints * c.nlayers * 4
+               + 2 * S.spectrum.nharm * S.grid.nlat_half * 4 for S, c, _, _ in items)
     res = {"batch_ms": dt_batch * 1e3, "transforms_per_s": 64.0 / dt_batch, "sequential_ms": dt_seq * 1e3,
            "speedup_vs_sequential": dt_seq / dt_batch, "uses_graph": B.uses_graph,
-           "total_columns": int(sum(it[1].nlayers for it in items)), "autotuned": tuned}
+           "total_columns": int(sum(it[1].nlayers for it in items)), "compulsory_bytes": b_rt,
+           "roofline_frac": b_rt / dt_batch / HBM_PEAK, "autotuned": tuned}
     B.close()
     for S in plans.values():
         S.close()

@@ -1944,6 +1944,7 @@ int build_batch(sht_batch *b, const sht_batch_item *items, int n, int flags) {
     std::vector<sht_plan *> plans;
     std::vector<CallArgs> calls;
     std::vector<LaunchUnit> st_units[4];
+    std::map<long long, size_t> leg_unit;  // (plan, stage, m) -> its Legendre unit
     int rc;
     // scratch: one Fourier buffer per item (a round trip's two calls share it)
     std::vector<size_t> fb_off(n);
@@ -1977,9 +1978,21 @@ int build_batch(sht_batch *b, const sht_batch_item *items, int n, int flags) {
                     st_units[st].push_back(u);
                 }
         };
+        // Legendre units of items on one plan merge per order m: the tiles of all those calls
+        // that read Λ_m run back to back on one XCD, so Λ_m comes from HBM once per batch
         auto add_leg = [&](int st, int call, const std::vector<Tile> &t, const std::vector<double> &c) {
-            const std::vector<LaunchUnit> u = legendre_units(t, c, call, cgs);
-            st_units[st].insert(st_units[st].end(), u.begin(), u.end());
+            for (const LaunchUnit &u : legendre_units(t, c, call, cgs)) {
+                const long long key = ((long long)pi * 4 + st) * 65536 + u.tiles[0].m;
+                auto f = leg_unit.find(key);
+                if (f == leg_unit.end()) {
+                    leg_unit[key] = st_units[st].size();
+                    st_units[st].push_back(u);
+                } else {
+                    LaunchUnit &v = st_units[st][f->second];
+                    v.cost += u.cost;
+                    v.tiles.insert(v.tiles.end(), u.tiles.begin(), u.tiles.end());
+                }
+            }
         };
         if (it.direction == SHT_DIR_SYNTHESIS || it.direction == SHT_DIR_ROUNDTRIP) {
             void *grid = it.direction == SHT_DIR_ROUNDTRIP ? it.mid : it.out;
