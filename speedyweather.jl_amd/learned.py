@@ -131,12 +131,15 @@ def forecast_score(net, nx, ny, seeds, spinup_hours=48.0, window_hours=1.0, memb
 # How the correction couples into the model (corrector keyword arguments): all of its
 # divergence, none of it, or only the divergence's departure from the column mean (the column
 # mean drives the surface-pressure tendency through continuity), with or without its own
-# ln ps tendency.
+# ln ps tendency, with the temperature correction's column mean or without it, or without
+# the temperature correction.
 CORRECTOR_VARIANTS = {
     "uvT": {"fields": ("vor", "div", "temp")},
     "vorT": {"fields": ("vor", "temp")},
     "uvT_baroclinic_div": {"fields": ("vor", "div", "temp"), "baroclinic_div": True},
     "uvTq_baroclinic_div": {"fields": ("vor", "div", "temp", "lnps"), "baroclinic_div": True},
+    "uvT_baroclinic": {"fields": ("vor", "div", "temp"), "baroclinic_div": True, "baroclinic_temp": True},
+    "uv_baroclinic_div": {"fields": ("vor", "div"), "baroclinic_div": True},
 }
 
 
@@ -216,12 +219,14 @@ def ab_score(net, nx, ny, x, y):
     return out
 
 
-def corrector(net, nx, ny, fields=("vor", "div", "temp"), baroclinic_div=False):
+def corrector(net, nx, ny, fields=("vor", "div", "temp"), baroclinic_div=False, baroclinic_temp=False):
     """A `correction` callback for DryPrimitiveModel.step: adds the MLP's column tendencies
     (u, v -> vor, div via curl / divergence of (u, v) / cos(lat); T; ln ps) to the
     spectral tendencies of `fields`.  ln ps is left to the plain tendency by default.
     baroclinic_div: the divergence correction minus its mean over the (equally spaced)
-    sigma levels, so that it does not change the surface-pressure tendency."""
+    sigma levels, so that it does not change the surface-pressure tendency.
+    baroclinic_temp: the same for the temperature correction, so that it does not change
+    the column's mean temperature (whose geopotential drives the column-mean divergence)."""
     def apply(model, tend, grid):
         E, L, npts = model.E, model.L, model.grid.npoints
         x = _columns(_grid_state(grid))
@@ -243,7 +248,11 @@ def corrector(net, nx, ny, fields=("vor", "div", "temp"), baroclinic_div=False):
                 dv = (dv - dv.mean(dim=1, keepdim=True)).view(EL, -1)
             tend["div"] += dv
         if "temp" in fields:
-            tend["temp"] += s[2 * EL:3 * EL] * keep
+            tt = s[2 * EL:3 * EL] * keep
+            if baroclinic_temp:
+                tt = tt.view(E, L, -1)
+                tt = (tt - tt.mean(dim=1, keepdim=True)).view(EL, -1)
+            tend["temp"] += tt
         if "lnps" in fields:
             tq = s[3 * EL:] * keep
             tq[:, 0] = 0

@@ -78,21 +78,24 @@ def test_truncation_tendency_learned(mods):
     # and the corrected coarse model's one-hour forecast is closer to the truncated fine run
     fc = ln.forecast_score(net, nx, ny, list(range(700, 708)))
     assert fc["vor"] < 1.0 and fc["div"] < 1.0 and fc["temp"] < 1.0, fc
-    # the coupling variants: chosen on validation runs; the baroclinic divergence correction
-    # has no column mean (it leaves the surface-pressure tendency to the plain model)
+    # the coupling variants: chosen on validation runs; the baroclinic divergence and
+    # temperature corrections have no column mean (the surface-pressure tendency and the
+    # column's mean temperature are left to the plain model)
     name, kw, val = ln.choose_corrector(net, nx, ny, list(range(800, 808)))
     assert name in ln.CORRECTOR_VARIANTS and set(val) == set(ln.CORRECTOR_VARIANTS)
     m = dy.DryPrimitiveModel(31, 8, nmembers=2)
     m.set_state(m.jet_state([3, 4]))
     tend, g = m.tendencies(m.state, with_grid=True)
     t0 = {k: v.clone() for k, v in tend.items()}
-    ln.corrector(net, nx, ny, baroclinic_div=True)(m, tend, g)
-    dd = (tend["div"] - t0["div"]).view(2, 8, -1)
-    # FP32: the difference carries the rounding of the sums t0 + correction (half an ulp of
-    # the tendency) besides the round-off of the correction's own mean removal
-    tol = 1e-6 * (float(t0["div"].abs().max()) + float(dd.abs().max()))
-    assert float(dd.mean(dim=1).abs().max()) <= tol
-    assert float(dd.abs().max()) > 0 and torch.equal(tend["lnps"], t0["lnps"])
+    ln.corrector(net, nx, ny, baroclinic_div=True, baroclinic_temp=True)(m, tend, g)
+    for f in ("div", "temp"):
+        dd = (tend[f] - t0[f]).view(2, 8, -1)
+        # FP32: the difference carries the rounding of the sums t0 + correction (half an ulp
+        # of the tendency) besides the round-off of the correction's own mean removal
+        tol = 1e-6 * (float(t0[f].abs().max()) + float(dd.abs().max()))
+        assert float(dd.mean(dim=1).abs().max()) <= tol, f
+        assert float(dd.abs().max()) > 0, f
+    assert torch.equal(tend["lnps"], t0["lnps"])
     m.close()
     # a corrected coarse run stays bounded
     m = dy.DryPrimitiveModel(31, 8, nmembers=2)
