@@ -91,6 +91,20 @@ __device__ inline void stage4(unsigned char *sm, int buf, int p, int kk, int g, 
     }
 }
 
+// Complex column (of the workgroup's 32) that staging thread tid stages: each group of 8
+// consecutive lanes q = (tid / 8) % 8 of wave w takes one column, 8 w + f(q), with f chosen
+// so that the tile rows written by one LDS write instruction fall on disjoint banks (64 banks
+// of 4 bytes, 256 bytes per cycle).  Split modes: 8-byte writes of 32 lanes per cycle, four
+// columns 2 f(q) whose 80-byte rows must start 64 bytes apart modulo 256: f = 0, 2, 4, 6
+// for q = 0..3 and 1, 3, 5, 7 for q = 4..7.  FP32: 16-byte writes of 16 lanes per cycle, two
+// columns whose 144-byte rows must start 128 bytes apart modulo 256: f = 0, 4, 1, 5, ...
+template <int MODE>
+__device__ __forceinline__ int stage_column(int tid) {
+    const int q = (tid >> 3) & 7;
+    const int f = MODE == kFP32 ? (q >> 1) + 4 * (q & 1) : 2 * (q & 3) + (q >> 2);
+    return (tid >> 6) * 8 + f;
+}
+
 // acc[t] += A_t (16 rows x 32 contraction, this lane's fragment of row tile t < TPW) x B
 // tile columns cb*16..cb*16+15 of parity p; each B fragment is read once for all TPW row
 // tiles.  FP32: af[t][8] holds the lane's A values of tile t for the 8 K-steps of 4.
@@ -170,7 +184,7 @@ __device__ inline void mma_chunk1(const unsigned char *sm, int buf, int p, int r
 // Workgroup = (order m, group of 64 rings, 32 complex columns).  Wave w owns ring tile
 // 4*blk+w (16 rings) and all 64 real columns, for both parities: 8 accumulators.
 // Degrees l = m + 2l' + p stream in chunks of 32 l' per parity; a staging thread loads 8
-// consecutive degrees of one complex column (kk = tid/8, degrees 8(tid%8)..+7).
+// consecutive degrees of one complex column (kk = stage_column(tid), degrees 8(tid%8)..+7).
 
 
 // Per-ring epilogue factors of ring rb < 64 of a synthesis block: 16 bytes in the padding of
@@ -220,7 +234,7 @@ __global__ __launch_bounds__(256, MODE == kBF16x3 ? 2 : 4) void legendre_synthes
     const int kbase = cg * kColsComplex;
     const int row = lane & 15, grp = lane >> 4;
     // staging role
-    const int kk = tid >> 3, g = tid & 7;
+    const int kk = stage_column<MODE>(tid), g = tid & 7;
     const bool kok = kbase + kk < K;
     const SHT_GLOBAL float2 *csrc = coeffs + (long long)(kok ? kbase + kk : 0) * ld_spec + off;
 
@@ -370,7 +384,7 @@ __global__ __launch_bounds__(256, MODE == kBF16x3 ? 2 : 4) void legendre_synthes
 // parity p = w&1 and TPW consecutive degree tiles (16 rows of l' each, l = m + 2l' + p) for
 // all 64 real columns, so the staged Fourier rows of a chunk serve DPW degrees.  Rings
 // stream in chunks of 32; a staging thread loads 4 consecutive rings of one complex column
-// (kk = tid/8, rings 4(tid%8)..+3).  Every output coefficient is written exactly once (no
+// (kk = stage_column(tid), rings 4(tid%8)..+3).  Every output coefficient is written exactly once (no
 // atomics, deterministic).
 
 // At least three workgroups per CU (at most 168 registers per lane).
@@ -404,7 +418,7 @@ __global__ __launch_bounds__(256, MODE == kBF16x3 ? 2 : 3) void legendre_analysi
     const long long off = as_global(P.lm_off)[m];
     const int kbase = cg * kColsComplex;
     const int row = lane & 15, grp = lane >> 4;
-    const int kk = tid >> 3, g = tid & 7;
+    const int kk = stage_column<MODE>(tid), g = tid & 7;
     const bool kok = kbase + kk < K;
     const int kgl = kok ? kbase + kk : 0;
 
