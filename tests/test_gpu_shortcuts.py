@@ -54,3 +54,40 @@ def test_shortcut_transforms_vs_oracle(sw, gpu, kind, n, lmax, mmax, shortcut, p
     sw.spec2grid_pullback_(cb, gb, S, accumulate=False)
     assert rel(cb.data.cpu().numpy(), o.spec2grid_pullback(g.astype(np.float32).astype(np.float64))) < TOL[precision]
     S.close()
+
+
+def test_shortcut_plans_in_batches_and_graph_replay(sw, gpu):
+    """A batch that mixes a plan with a non-monotonic shortcut (full grid, LinCubCoslat) and a
+    linear-shortcut plan, and graph replay of single calls on the shortcut plan, equal direct
+    single calls bitwise (the zero Fourier rows up to the running maximum included)."""
+    K = 5
+    sp = sw.Spectrum(33, 32)
+    Sf = sw.SpectralTransform(sp, sw.FullGaussianGrid(24), nlayers=K, LegendreShortcut="lincubcoslat")
+    So = sw.SpectralTransform(sp, sw.OctahedralGaussianGrid(24), nlayers=K)
+    items, expect = [], []
+    for i, S in enumerate((Sf, So)):
+        a = sw.seeded_spectrum(sp, K, seed=40 + i)
+        c = sw.LowerTriangularArray(torch.from_numpy(a).to(torch.complex64).cuda(), sp)
+        mid = sw.Field(torch.zeros((K, S.grid.npoints), device="cuda"), S.grid)
+        out = sw.LowerTriangularArray.zeros(sp, K, device="cuda")
+        items.append((S, c, out, mid))
+        f = sw.transform(c, S)
+        expect.append((f.data.clone(), sw.transform(f, S).data.clone()))
+    B = sw.SpectralBatch(items)
+    B.launch()
+    torch.cuda.synchronize()
+    for (S, c, out, mid), (f, b) in zip(items, expect):
+        assert torch.equal(mid.data, f) and torch.equal(out.data, b)
+    B.close()
+    Sf.set_graphs(True)
+    c = items[0][1]
+    f = sw.Field(torch.zeros((K, Sf.grid.npoints), device="cuda"), Sf.grid)
+    b = sw.LowerTriangularArray.zeros(sp, K, device="cuda")
+    for rep in range(3):
+        sw.transform_(f, c, Sf)
+        sw.transform_(b, f, Sf)
+        torch.cuda.synchronize()
+        assert torch.equal(f.data, expect[0][0]) and torch.equal(b.data, expect[0][1]), rep
+    assert Sf.graph_count() == (2, 0)
+    Sf.close()
+    So.close()
