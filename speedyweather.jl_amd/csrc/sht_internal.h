@@ -73,10 +73,20 @@ constexpr int split_parts(int mode) { return mode == kBF16x3 ? 3 : 2; }
 // group (analysis), column group cg.  FFT stages: m = ring, blk = column group.  call
 // indexes the launch's call arguments (always 0 for a single transform).
 struct Tile {
-    int call;
-    int m;  // < 0: padding, the workgroup exits at once
+    int call;  // batch Legendre tiles: < 0 for merged column group -1 - call (see ColRef)
+    int m;     // < 0: padding, the workgroup exits at once
     int blk;
     int cg;
+};
+
+// A column of a merged Legendre column group: batch items on one plan (and, for the
+// synthesis, one unscale_coslat flag) share column groups of kColsComplex columns, so one
+// workgroup (order m, ring or degree block) reads Λ once for all of them.  Group q's columns
+// are entries q kColsComplex .. + kColsComplex - 1 of the batch's column map, which batch
+// launches pass as single.in; call < 0 marks an empty column.
+struct ColRef {
+    int call;
+    int col;
 };
 
 // Workgroups go to the 8 XCDs (each with its own L2) round-robin by linear block index.

@@ -105,6 +105,23 @@ __device__ __forceinline__ int stage_column(int tid) {
     return (tid >> 6) * 8 + f;
 }
 
+// Batch launches: the column map of merged column groups (passed as single.in) and the
+// entry of column kk of tile T, or null / an empty entry for a tile of one call.
+template <bool BATCH>
+__device__ __forceinline__ const SHT_GLOBAL ColRef *merged_columns(const Tile &T, const CallArgs &single) {
+    if (!BATCH || T.call >= 0) return nullptr;
+    return as_global(static_cast<const ColRef *>(single.in)) + (long long)(-1 - T.call) * kColsComplex;
+}
+__device__ __forceinline__ ColRef column_ref(const SHT_GLOBAL ColRef *cmap, int k) {
+    const SHT_GLOBAL int *q = reinterpret_cast<const SHT_GLOBAL int *>(cmap + k);
+    return ColRef{q[0], q[1]};
+}
+// The call of a merged column (an empty one reads the group's first call: same plan and
+// direction, so every index stays inside its buffers; nothing is stored for it).
+__device__ __forceinline__ const SHT_GLOBAL CallArgs *call_of(const CallArgs *calls, int call, int call0) {
+    return as_global(calls) + (call < 0 ? call0 : call);
+}
+
 // acc[t] += A_t (16 rows x 32 contraction, this lane's fragment of row tile t < TPW) x B
 // tile columns cb*16..cb*16+15 of parity p; each B fragment is read once for all TPW row
 // tiles.  FP32: af[t][8] holds the lane's A values of tile t for the 8 K-steps of 4.
@@ -214,7 +231,9 @@ __global__ __launch_bounds__(256, MODE == kBF16x3 ? 2 : 4) void legendre_synthes
     const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const Tile T = tiles[blockIdx.x];
     if (T.m < 0) return;  // padding of the launch order
-    const CallArgs C = BATCH ? calls[T.call] : single;
+    const SHT_GLOBAL ColRef *cmap = merged_columns<BATCH>(T, single);
+    const int call0 = cmap ? column_ref(cmap, 0).call : T.call;
+    const CallArgs C = BATCH ? calls[call0] : single;
     const DevPlan &P = plans[C.plan];
     const int m = T.m, cg = T.cg;
     const SHT_GLOBAL float2 *coeffs = as_global(static_cast<const float2 *>(C.in));
@@ -235,8 +254,14 @@ __global__ __launch_bounds__(256, MODE == kBF16x3 ? 2 : 4) void legendre_synthes
     const int row = lane & 15, grp = lane >> 4;
     // staging role
     const int kk = stage_column<MODE>(tid), g = tid & 7;
-    const bool kok = kbase + kk < K;
+    bool kok = kbase + kk < K;
     const SHT_GLOBAL float2 *csrc = coeffs + (long long)(kok ? kbase + kk : 0) * ld_spec + off;
+    if (BATCH && cmap) {  // the column's own call
+        const ColRef r = column_ref(cmap, kk);
+        const SHT_GLOBAL CallArgs *ck = call_of(calls, r.call, call0);
+        kok = r.call >= 0;
+        csrc = as_global(static_cast<const float2 *>(ck->in)) + (long long)(kok ? r.col : 0) * ck->ld_in + off;
+    }
 
     __shared__ alignas(16) unsigned char sm[LegSmem<MODE>::bytes];
 
@@ -353,7 +378,17 @@ __global__ __launch_bounds__(256, MODE == kBF16x3 ? 2 : 4) void legendre_synthes
     for (int cb = 0; cb < 4; ++cb) {
         const int col = cb * 16 + row;
         const int kc = col >> 1, reim = col & 1;
-        const int k = kbase + kc;
+        int k = kbase + kc, Kk = K;
+        bool kin = k < K;
+        SHT_GLOBAL float4 *fb = fbuf;
+        if (BATCH && cmap) {  // the column's own call
+            const ColRef r = column_ref(cmap, kc);
+            const SHT_GLOBAL CallArgs *ck = call_of(calls, r.call, call0);
+            kin = r.call >= 0;
+            k = r.col;
+            Kk = ck->K;
+            fb = as_global(static_cast<float4 *>(ck->fbuf));
+        }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int r = 4 * grp + i;
@@ -365,9 +400,9 @@ __global__ __launch_bounds__(256, MODE == kBF16x3 ? 2 : 4) void legendre_synthes
             const float nv = e + o, sv = e - o;
             const float n_other = lane_xor1(nv);
             const float s_other = lane_xor1(sv);
-            if (j < n && k < K) {
+            if (j < n && kin) {
                 SHT_GLOBAL float2 *dst = reinterpret_cast<SHT_GLOBAL float2 *>(
-                    fbuf + *reinterpret_cast<const long long *>(slot) * (long long)K + k);
+                    fb + *reinterpret_cast<const long long *>(slot) * (long long)Kk + k);
                 if (reim == 0) {
                     gstore(dst, make_float2(nv, n_other));
                 } else {
@@ -399,7 +434,9 @@ __global__ __launch_bounds__(256, MODE == kBF16x3 ? 2 : 3) void legendre_analysi
     const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const Tile T = tiles[blockIdx.x];
     if (T.m < 0) return;  // padding of the launch order
-    const CallArgs C = BATCH ? calls[T.call] : single;
+    const SHT_GLOBAL ColRef *cmap = merged_columns<BATCH>(T, single);
+    const int call0 = cmap ? column_ref(cmap, 0).call : T.call;
+    const CallArgs C = BATCH ? calls[call0] : single;
     const DevPlan &P = plans[C.plan];
     const int m = T.m, cg = T.cg;
     const SHT_GLOBAL float4 *fbuf = as_global(static_cast<const float4 *>(C.fbuf));
@@ -419,8 +456,17 @@ __global__ __launch_bounds__(256, MODE == kBF16x3 ? 2 : 3) void legendre_analysi
     const int kbase = cg * kColsComplex;
     const int row = lane & 15, grp = lane >> 4;
     const int kk = stage_column<MODE>(tid), g = tid & 7;
-    const bool kok = kbase + kk < K;
-    const int kgl = kok ? kbase + kk : 0;
+    bool kok = kbase + kk < K;
+    int kgl = kok ? kbase + kk : 0, sK = K;  // staging column, its Fourier scratch and row length
+    const SHT_GLOBAL float4 *sfb = fbuf;
+    if (BATCH && cmap) {  // the column's own call
+        const ColRef r = column_ref(cmap, kk);
+        const SHT_GLOBAL CallArgs *ck = call_of(calls, r.call, call0);
+        kok = r.call >= 0;
+        kgl = kok ? r.col : 0;
+        sK = ck->K;
+        sfb = as_global(static_cast<const float4 *>(ck->fbuf));
+    }
 
     constexpr int OUTB = kColsComplex * (DPW + 1) * 8;
     constexpr int SMB = LegSmem<MODE>::bytes > OUTB ? LegSmem<MODE>::bytes : OUTB;
@@ -477,7 +523,7 @@ __global__ __launch_bounds__(256, MODE == kBF16x3 ? 2 : 3) void legendre_analysi
         c = min(c, JC - 1);
         load_offsets(S.fo, c + 1);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) S.v[i] = gload(fbuf + (long long)(fo[i] + m) * K + kgl);
+        for (int i = 0; i < 4; ++i) S.v[i] = gload(sfb + (long long)(fo[i] + m) * sK + kgl);
 #pragma unroll
         for (int t = 0; t < TPW; ++t) {
             if constexpr (MODE == kFP32) {
@@ -542,13 +588,25 @@ __global__ __launch_bounds__(256, MODE == kBF16x3 ? 2 : 3) void legendre_analysi
         }
     __syncthreads();
     constexpr int NIT = kColsComplex * DPW / 256;
-    if (!C.accumulate && !C.lap) {  // plain transform (uniform)
+    if (!C.accumulate && !C.lap) {  // plain transform (uniform); batch calls are plain
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
             const int idx = it * 256 + tid;
             const int kc = idx / DPW, o = idx % DPW;
-            const int og = DPW * T.blk + o, kq = kbase + kc;
-            if (og < L && kq < K) gstore(coeffs + (long long)kq * ld_spec + off + og, Out[kc * (DPW + 1) + o]);
+            const int og = DPW * T.blk + o;
+            int kq = kbase + kc;
+            bool kin = kq < K;
+            SHT_GLOBAL float2 *dst = coeffs;
+            long long ld = ld_spec;
+            if (BATCH && cmap) {  // the column's own call
+                const ColRef r = column_ref(cmap, kc);
+                const SHT_GLOBAL CallArgs *ck = call_of(calls, r.call, call0);
+                kin = r.call >= 0;
+                kq = r.col;
+                dst = as_global(static_cast<float2 *>(ck->out));
+                ld = ck->ld_out;
+            }
+            if (og < L && kin) gstore(dst + (long long)kq * ld + off + og, Out[kc * (DPW + 1) + o]);
         }
         return;
     }

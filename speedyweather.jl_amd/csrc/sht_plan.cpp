@@ -24,6 +24,7 @@
 
 using sht::CallArgs;
 using sht::DevPlan;
+using sht::ColRef;
 using sht::Tile;
 
 namespace {
@@ -1917,6 +1918,7 @@ struct sht_batch {
     std::vector<void *> allocs;
     const DevPlan *plans = nullptr;
     const CallArgs *calls = nullptr;
+    const ColRef *colmap = nullptr;  // merged Legendre column groups, passed as single.in
     const Tile *tiles[4] = {nullptr, nullptr, nullptr, nullptr};
     int ntiles[4] = {0, 0, 0, 0};
     unsigned fft_smem = 0;
@@ -1944,7 +1946,8 @@ int build_batch(sht_batch *b, const sht_batch_item *items, int n, int flags) {
     std::vector<sht_plan *> plans;
     std::vector<CallArgs> calls;
     std::vector<LaunchUnit> st_units[4];
-    std::map<long long, size_t> leg_unit;  // (plan, stage, m) -> its Legendre unit
+    std::map<std::tuple<int, int, int>, std::vector<int>> leg_calls;  // (plan, stage, unscale) -> calls
+    std::vector<ColRef> colmap;                                     // merged column groups
     int rc;
     // scratch: one Fourier buffer per item (a round trip's two calls share it)
     std::vector<size_t> fb_off(n);
@@ -1978,21 +1981,10 @@ int build_batch(sht_batch *b, const sht_batch_item *items, int n, int flags) {
                     st_units[st].push_back(u);
                 }
         };
-        // Legendre units of items on one plan merge per order m: the tiles of all those calls
-        // that read Λ_m run back to back on one XCD, so Λ_m comes from HBM once per batch
-        auto add_leg = [&](int st, int call, const std::vector<Tile> &t, const std::vector<double> &c) {
-            for (const LaunchUnit &u : legendre_units(t, c, call, cgs)) {
-                const long long key = ((long long)pi * 4 + st) * 65536 + u.tiles[0].m;
-                auto f = leg_unit.find(key);
-                if (f == leg_unit.end()) {
-                    leg_unit[key] = st_units[st].size();
-                    st_units[st].push_back(u);
-                } else {
-                    LaunchUnit &v = st_units[st][f->second];
-                    v.cost += u.cost;
-                    v.tiles.insert(v.tiles.end(), u.tiles.begin(), u.tiles.end());
-                }
-            }
+        // Legendre calls are grouped by plan (and unscale flag): after the loop, the calls of a
+        // group share column groups (merged_groups below)
+        auto add_leg = [&](int st, int call, const std::vector<Tile> &, const std::vector<double> &) {
+            leg_calls[{pi, st, calls[call].unscale}].push_back(call);
         };
         if (it.direction == SHT_DIR_SYNTHESIS || it.direction == SHT_DIR_ROUNDTRIP) {
             void *grid = it.direction == SHT_DIR_ROUNDTRIP ? it.mid : it.out;
@@ -2011,6 +2003,46 @@ int build_batch(sht_batch *b, const sht_batch_item *items, int n, int flags) {
             add_leg(sht::kLegendreAnalysis, call, pl->h_ana, pl->h_ana_cost);
         }
     }
+    // Legendre units.  The calls of one (plan, stage, unscale) group fill column groups of
+    // kColsComplex columns in call order, so a workgroup (order m, ring or degree block) reads
+    // Λ once for the columns of several items; a group of one call keeps its own column
+    // groups.  Units gather the tiles of one order m of a group, so they run back to back
+    // on one XCD.
+    for (const auto &kv : leg_calls) {
+        const int pi = std::get<0>(kv.first), st = std::get<1>(kv.first);
+        const sht_plan *pl = plans[pi];
+        const std::vector<Tile> &t = st == sht::kLegendreSynthesis ? pl->h_syn : pl->h_ana;
+        const std::vector<double> &c = st == sht::kLegendreSynthesis ? pl->h_syn_cost : pl->h_ana_cost;
+        const std::vector<int> &cl = kv.second;
+        if (cl.size() == 1) {
+            const int call = cl[0], cgs = (calls[call].K + sht::kColsComplex - 1) / sht::kColsComplex;
+            for (const LaunchUnit &u : legendre_units(t, c, call, cgs)) st_units[st].push_back(u);
+            continue;
+        }
+        const int g0 = (int)(colmap.size() / sht::kColsComplex);
+        for (int call : cl)
+            for (int k = 0; k < calls[call].K; ++k) colmap.push_back({call, k});
+        while (colmap.size() % sht::kColsComplex) colmap.push_back({-1, 0});
+        const int ng = (int)(colmap.size() / sht::kColsComplex) - g0;
+        std::vector<LaunchUnit> units;
+        std::vector<int> unit_of_m;
+        for (size_t q = 0; q < t.size(); ++q) {
+            const int m = t[q].m;
+            if (m >= (int)unit_of_m.size()) unit_of_m.resize(m + 1, -1);
+            if (unit_of_m[m] < 0) {
+                unit_of_m[m] = (int)units.size();
+                units.push_back({0.0, {}});
+            }
+            LaunchUnit &u = units[unit_of_m[m]];
+            for (int g = 0; g < ng; ++g) {
+                u.tiles.push_back({-1 - (g0 + g), m, t[q].blk, 0});
+                u.cost += c[q];
+            }
+        }
+        st_units[st].insert(st_units[st].end(), units.begin(), units.end());
+    }
+    if (colmap.empty()) colmap.push_back({-1, 0});
+    if ((rc = dev_upload(b->allocs, colmap, &b->colmap))) return rc;
     std::vector<DevPlan> dplans;
     for (sht_plan *pl : plans) {
         dplans.push_back(pl->P);
@@ -2034,6 +2066,7 @@ int build_batch(sht_batch *b, const sht_batch_item *items, int n, int flags) {
     if (hipGraphCreate(&b->graph, 0) != hipSuccess) return SHT_OK;
     hipGraphNode_t prev = nullptr;
     CallArgs none{};
+    none.in = b->colmap;
     for (int st = 0; st < 4; ++st) {
         if (b->ntiles[st] == 0) continue;
         const DevPlan *a0 = b->plans;
@@ -2115,6 +2148,7 @@ int sht_batch_launch(sht_batch *b, void *stream) {
         return SHT_OK;
     }
     CallArgs none{};
+    none.in = b->colmap;
     for (int st = 0; st < 4; ++st) {
         if (b->ntiles[st] == 0) continue;
         const unsigned smem = (st == sht::kFFTSynthesis || st == sht::kFFTAnalysis) ? b->fft_smem : 0;
