@@ -384,8 +384,11 @@ def summarize_counters(paths):
     are known), MFMA busy share (SQ_VALU_MFMA_BUSY_CYCLES, summed over the 1024 SIMDs, over
     1024 x the kernel's cycles; GRBM_GUI_ACTIVE sums the 8 XCDs' cycles: it reads 22.7 to 23.4
     per ns of kernel time on MI355X, 8 x 2.4 GHz plus the dispatch),
-    MFMA flops (SQ_INSTS_VALU_MFMA_MOPS_* x 512) and the share of LDS-active cycles stalled by
-    bank conflicts (SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE)."""
+    MFMA flops (SQ_INSTS_VALU_MFMA_MOPS_* x 512), the share of LDS-active cycles stalled by
+    bank conflicts (SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE), and where the waves' time goes,
+    as shares of SQ_WAVE_CYCLES: issuing any instruction, VALU, LDS or scalar instructions
+    (SQ_ACTIVE_INST_ANY / _VALU / _LDS / _SCA), waiting for a dependency (s_waitcnt,
+    SQ_WAIT_ANY) and waiting for issue (SQ_WAIT_INST_ANY)."""
     import csv
     per = {}   # dispatch -> {"row": row, counters...}
     scale = {}
@@ -410,7 +413,7 @@ def summarize_counters(paths):
     lines = [f"# copy calibration: bytes = FETCH_SIZE x 1024 x {scale.get('FETCH_SIZE', float('nan')):.4f}, "
              f"WRITE_SIZE x 1024 x {scale.get('WRITE_SIZE', float('nan')):.4f}",
              "# kernel precision fetch_bytes write_bytes launches workgroups mfma_busy_pct mfma_gflop "
-             "lds_conflict_pct_of_active"]
+             "lds_conflict_pct_of_active issue_any valu lds salu wait_dep wait_issue"]
     for (kernel, mode), byshape in sorted(groups.items()):
         (wg, vgpr, lds), ds = max(byshape.items(), key=lambda kv: len(kv[1]))
 
@@ -423,7 +426,10 @@ def summarize_counters(paths):
         flop = 512 * sum(mean(c) for c in ("SQ_INSTS_VALU_MFMA_MOPS_F16", "SQ_INSTS_VALU_MFMA_MOPS_BF16",
                                              "SQ_INSTS_VALU_MFMA_MOPS_F32")) * 1e-9
         conf = 100 * mean("SQ_LDS_BANK_CONFLICT") / mean("SQ_LDS_IDX_ACTIVE")
-        lines.append(f"{kernel} {mode} {fetch:.0f} {write:.0f} {len(ds)} {wg} {busy:.1f} {flop:.3f} {conf:.1f}")
+        wc = mean("SQ_WAVE_CYCLES")
+        shares = " ".join(f"{mean(c) / wc:.2f}" for c in ("SQ_ACTIVE_INST_ANY", "SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_LDS",
+                                                          "SQ_ACTIVE_INST_SCA", "SQ_WAIT_ANY", "SQ_WAIT_INST_ANY"))
+        lines.append(f"{kernel} {mode} {fetch:.0f} {write:.0f} {len(ds)} {wg} {busy:.1f} {flop:.3f} {conf:.1f} {shares}")
     return "\n".join(lines)
 
 
