@@ -437,16 +437,19 @@ __device__ __forceinline__ FFTPass load_pass(const SHT_GLOBAL FFTPass *p) {
 // Runs all passes in place on buf (kFFTCols columns of N points, the first ncol of them
 // data): a first-pass MFMA radix, then the register passes, all across the workgroup.
 template <bool INV>
-__device__ __attribute__((always_inline)) void fft_run(const DevPlan &P, float2 *buf, const float2 *tw,
-                                                        const SHT_GLOBAL FFTPass *passes, int nf, int N, int mt,
-                                                        int ncol, int tid) {
+__device__ __attribute__((always_inline)) void fft_run(const DevPlan &P, const FFTRing &R, float2 *buf,
+                                                        const float2 *tw, int N, int mt, int ncol, int tid) {
+    const SHT_GLOBAL FFTPass *passes = as_global(P.passes) + R.passes;
+    const int nf = R.nf;
     int f = 0;
-    // an odd first radix of 11 or more is a prime the plan placed first for the MFMA pass
-    // (register passes of 11 and 13 come last; 16 is the only even register radix above 8)
-    if (nf > 0 && load_pass(passes).R >= kMfmaMinPrime && (load_pass(passes).R & 1)) {
-        const FFTPass p0 = load_pass(passes);
-        const SHT_GLOBAL b8 *tab = reinterpret_cast<const SHT_GLOBAL b8 *>(as_global(P.prime_tab)) +
-                                   as_global(P.prime_off)[p0.R];
+    if (R.prime) {  // the MFMA first pass (its descriptor and table offset come with the ring)
+        FFTPass p0;
+        p0.R = R.prime;
+        p0.l = 1;
+        p0.M = R.M0;
+        p0.mulM = R.mulM0;
+        p0.mulR = R.mulR0;
+        const SHT_GLOBAL b8 *tab = reinterpret_cast<const SHT_GLOBAL b8 *>(as_global(P.prime_tab)) + R.prime_off;
         const int h = (p0.R - 1) >> 1;
         const int tmax = INV ? min(h, (mt + p0.M - 1) / p0.M) : h;
         fft_pass_mfma_first<INV>(buf, tab, N, p0, ncol, max(tmax, 1), tid);
@@ -469,8 +472,8 @@ __device__ __attribute__((always_inline)) void fft_run(const DevPlan &P, float2 
 }
 
 // Twiddles of ring j into LDS with all loads of a thread in flight together.
-__device__ inline void load_twiddles(const DevPlan &P, int j, int N, float2 *tw, int tid) {
-    const SHT_GLOBAL float2 *gtw = as_global(reinterpret_cast<const float2 *>(P.twiddle)) + as_global(P.tw_off)[j];
+__device__ inline void load_twiddles(const DevPlan &P, long long tw_off, int N, float2 *tw, int tid) {
+    const SHT_GLOBAL float2 *gtw = as_global(reinterpret_cast<const float2 *>(P.twiddle)) + tw_off;
     constexpr int U = (kFFTMaxRing + kFFTThreads - 1) / kFFTThreads;
     float2 v[U];
 #pragma unroll
@@ -486,33 +489,56 @@ __device__ inline void load_twiddles(const DevPlan &P, int j, int N, float2 *tw,
     }
 }
 
-// Per-ring launch geometry of the FFT kernels: ring j, first column k0, columns ncol.
-struct FFTTile {
-    int j, N, mt, k0, ncol;
-};
+// The ring record of a rank, read field by field (structures do not copy through global
+// pointers): one level of loads after the rank is known.
+__device__ __forceinline__ FFTRing load_ring(const SHT_GLOBAL FFTRing *p) {
+    const SHT_GLOBAL int *q = reinterpret_cast<const SHT_GLOBAL int *>(p);
+    const SHT_GLOBAL long long *w = reinterpret_cast<const SHT_GLOBAL long long *>(p);
+    FFTRing r;
+    r.j = q[0];
+    r.N = q[1];
+    r.mt = q[2];
+    r.mte = q[3];
+    r.nf = q[4];
+    r.prime = q[5];
+    r.M0 = q[6];
+    r.mulM0 = (unsigned)q[7];
+    r.mulR0 = (unsigned)q[8];
+    r.tw_off = w[6];
+    r.fboff = w[7];
+    r.ring_n = w[8];
+    r.ring_s = w[9];
+    r.prime_off = w[10];
+    r.passes = w[11];
+    return r;
+}
 
-// Last Fourier row of ring j the analysis writes: rows mt_j < m <= mte_j (orders that
-// rings nearer the pole keep under a non-monotonic shortcut) are written as zeros, since
-// the Legendre analysis sums every ring from the first one of an order on.
-__device__ inline int fft_rows_written(const DevPlan &P, int j) { return as_global(P.mte)[j]; }
+// Per-workgroup launch geometry of the FFT kernels: the ring's record, first column k0,
+// columns ncol.
+struct FFTTile {
+    FFTRing R;
+    int k0, ncol;
+};
 
 // Single-call launch orders, rings heaviest first.  Paired (synthesis, which reads the
 // Fourier rows): units (ring rank r, column-group pair p) go round-robin to the XCDs and
 // the two column groups of a unit, which share the 128-byte lines of the Fourier rows, run
 // back to back on the same XCD: block b -> XCD b % 8, slot s = b / 8, unit
 // (s / 2) * 8 + b % 8, column group 2 p + s % 2.  Otherwise (analysis): block b -> ring
-// rank b / kgs, column group b % kgs.  Returns false for blocks past the end.
+// rank b / kgs, column group b % kgs.  Batch tiles carry the rank (m) and column group.
+// rings: the plan's ring records (single calls pass them in place of a tile list).
+// Returns false for blocks past the end.
 template <bool BATCH>
-__device__ inline bool fft_tile(const DevPlan &P, const Tile &T, int K, bool paired, FFTTile &t) {
+__device__ inline bool fft_tile(const DevPlan &P, const SHT_GLOBAL FFTRing *rings, const Tile &T, int K, bool paired,
+                                FFTTile &t) {
     const int kgs = (K + kFFTCols - 1) / kFFTCols;
-    int cg;
+    int cg, r;
     if (BATCH) {
         if (T.m < 0) return false;
-        t.j = T.m;
+        r = T.m;
         cg = T.blk;
     } else {
         const int b = blockIdx.x;
-        int r;
         if (paired) {
             const int npair = (kgs + 1) >> 1;
             const int s = b / kXCDs;
@@ -524,10 +550,8 @@ __device__ inline bool fft_tile(const DevPlan &P, const Tile &T, int K, bool pai
             cg = b - r * kgs;
         }
         if (r >= P.n || cg >= kgs) return false;
-        t.j = as_global(P.ring_order)[r];
     }
-    t.N = as_global(P.nlon)[t.j];
-    t.mt = as_global(P.mt)[t.j];
+    t.R = load_ring(rings + r);
     t.k0 = cg * kFFTCols;
     t.ncol = min(kFFTCols, K - t.k0);
     return true;
@@ -553,15 +577,18 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_synthesis_kernel(const Dev
     const CallArgs C = BATCH ? calls[T.call] : single;
     const DevPlan &P = plans[C.plan];
     const int K = C.K;
+    const SHT_GLOBAL FFTRing *rings =
+        BATCH ? as_global(P.rings) : as_global(reinterpret_cast<const FFTRing *>(tiles));
     FFTTile G;
-    if (!fft_tile<BATCH>(P, T, K, C.paired != 0, G)) return;  // padding of the launch order
-    const int N = G.N, mt = G.mt, ncol = G.ncol;
+    if (!fft_tile<BATCH>(P, rings, T, K, C.paired != 0, G)) return;  // padding of the launch order
+    const FFTRing &R = G.R;
+    const int N = R.N, mt = R.mt, ncol = G.ncol;
     float2 *tw = smem;
     float2 *b0 = smem + N;
-    load_twiddles(P, G.j, N, tw, tid);
+    load_twiddles(P, R.tw_off, N, tw, tid);
     {
         const SHT_GLOBAL float4 *fb =
-            as_global(static_cast<const float4 *>(C.fbuf)) + as_global(P.fboff)[G.j] * (long long)K + G.k0;
+            as_global(static_cast<const float4 *>(C.fbuf)) + R.fboff * (long long)K + G.k0;
         const int half = N / 2;
         const int col = tid % kFFTCols, m0 = tid / kFFTCols;
         float4 v[kFFTRowPasses];
@@ -573,7 +600,7 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_synthesis_kernel(const Dev
         }
         __builtin_amdgcn_sched_barrier(0);  // every load in flight before the first LDS store
         if (C.adj) {  // adjoint of the analysis (grid2spec pullback): rows times ΔΩ_j / s_m
-            const float wj = as_global(P.domega)[G.j];
+            const float wj = as_global(P.domega)[R.j];
 #pragma unroll
             for (int u = 0; u < kFFTRowPasses; ++u) {
                 const int mm = m0 + kFFTRowsPerPass * u;
@@ -598,12 +625,12 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_synthesis_kernel(const Dev
         }
     }
     __syncthreads();
-    fft_run<true>(P, b0, tw, as_global(P.passes) + G.j * kMaxFactors, as_global(P.nfac)[G.j], N, mt, ncol, tid);
+    fft_run<true>(P, R, b0, tw, N, mt, ncol, tid);
     __syncthreads();
     {
         SHT_GLOBAL float *field = as_global(static_cast<float *>(C.out));
         const long long ld = C.ld_out;
-        const long long rn = as_global(P.ring_n)[G.j], rs = as_global(P.ring_s)[G.j];
+        const long long rn = R.ring_n, rs = R.ring_s;
         constexpr int U = (kFFTMaxRing + kFFTThreads - 1) / kFFTThreads;
 #pragma unroll
         for (int c = 0; c < kFFTCols; ++c) {
@@ -659,16 +686,19 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_analysis_kernel(const DevP
     const CallArgs C = BATCH ? calls[T.call] : single;
     const DevPlan &P = plans[C.plan];
     const int K = C.K;
+    const SHT_GLOBAL FFTRing *rings =
+        BATCH ? as_global(P.rings) : as_global(reinterpret_cast<const FFTRing *>(tiles));
     FFTTile G;
-    if (!fft_tile<BATCH>(P, T, K, false, G)) return;  // padding of the launch order
-    const int N = G.N, mt = G.mt, ncol = G.ncol;
+    if (!fft_tile<BATCH>(P, rings, T, K, false, G)) return;  // padding of the launch order
+    const FFTRing &R = G.R;
+    const int N = R.N, mt = R.mt, ncol = G.ncol;
     float2 *tw = smem;
     float2 *b0 = smem + N;
-    load_twiddles(P, G.j, N, tw, tid);
+    load_twiddles(P, R.tw_off, N, tw, tid);
     {
         const SHT_GLOBAL float *field = as_global(static_cast<const float *>(C.in));
         const long long ld = C.ld_in;
-        const long long rn = as_global(P.ring_n)[G.j], rs = as_global(P.ring_s)[G.j];
+        const long long rn = R.ring_n, rs = R.ring_s;
         constexpr int U = (kFFTMaxRing + kFFTThreads - 1) / kFFTThreads;
         // all loads of the thread in flight before the first LDS store (columns past ncol
         // re-read column 0 and are not stored)
@@ -696,15 +726,18 @@ __global__ __launch_bounds__(kFFTThreads, 5) void fft_analysis_kernel(const DevP
         }
     }
     __syncthreads();
-    fft_run<false>(P, b0, tw, as_global(P.passes) + G.j * kMaxFactors, as_global(P.nfac)[G.j], N, mt, ncol, tid);
+    fft_run<false>(P, R, b0, tw, N, mt, ncol, tid);
     __syncthreads();
     {
-        SHT_GLOBAL float4 *fb = as_global(static_cast<float4 *>(C.fbuf)) + as_global(P.fboff)[G.j] * (long long)K + G.k0;
+        SHT_GLOBAL float4 *fb = as_global(static_cast<float4 *>(C.fbuf)) + R.fboff * (long long)K + G.k0;
         const int col = tid % kFFTCols, m0 = tid / kFFTCols;
         const float2 *z = b0 + col * N;
         // adjoint of the synthesis (spec2grid pullback): rows times s_m (1/cos lat) / ΔΩ_j
-        const float wj = C.adj ? (C.unscale ? as_global(P.coslat_inv)[G.j] : 1.f) / as_global(P.domega)[G.j] : 1.f;
-        const int mte = fft_rows_written(P, G.j);
+        const float wj = C.adj ? (C.unscale ? as_global(P.coslat_inv)[R.j] : 1.f) / as_global(P.domega)[R.j] : 1.f;
+        // rows mt < m <= mte (orders that rings nearer the pole keep under a non-monotonic
+        // shortcut) are written as zeros: the Legendre analysis sums every ring from the
+        // first one of an order on
+        const int mte = R.mte;
 #pragma unroll
         for (int u = 0; u < kFFTRowPasses; ++u) {
             const int mm = m0 + kFFTRowsPerPass * u;

@@ -177,6 +177,24 @@ struct FFTPass {
 // FastDiv multiplier of d: x / d == umulhi(x, mul) for x, d < 2^16.
 inline unsigned fastdiv_mul(int d) { return d > 1 ? 0xFFFFFFFFu / (unsigned)d + 1u : 0u; }
 
+// One FFT workgroup rank's ring (launch order, heaviest first, per factorisation policy):
+// everything the FFT kernels read per ring in one record, so that a workgroup's first loads
+// depend on no other load.  prime, M0, mulM0, mulR0: the MFMA first pass (radix, butterflies
+// per column, FastDiv multipliers of M0 and prime), prime = 0 without one.
+struct FFTRing {
+    int j, N, mt, mte;
+    int nf, prime, M0;
+    unsigned mulM0, mulR0;
+    int pad[3];
+    long long tw_off;     // first twiddle of the ring
+    long long fboff;      // first Fourier-buffer row
+    long long ring_n;     // first grid point of the northern ring
+    long long ring_s;     // first grid point of the southern ring
+    long long prime_off;  // the MFMA first pass's table (DevPlan::prime_tab, 8-element vectors)
+    long long passes;     // first pass descriptor (index into DevPlan::passes)
+};
+static_assert(sizeof(FFTRing) == 96, "FFTRing layout");
+
 // Device view of a plan (kernels read it through a device pointer).
 struct DevPlan {
     int n;            // rings per hemisphere
@@ -197,6 +215,7 @@ struct DevPlan {
     const float *coslat_inv;
     const float *domega;      // solid angle ΔΩ_j of a grid point on ring j
     const int *ring_order;    // FFT workgroup -> ring, most expensive first
+    const FFTRing *rings;     // FFT workgroup rank -> its ring's record (same order)
 
     // per order m
     const int *j0;               // first ring with mt >= m (rings j0..n-1 contribute)

@@ -916,9 +916,9 @@ int build_plan(sht_plan *pl) {
         if ((rc = dev_upload(A, ana_f32, &P.ana_f32))) return rc;
     }
     if ((rc = dev_upload(A, tw, &P.twiddle))) return rc;
+    std::vector<long long> poff;  // MFMA prime tables: offset per radix
     {
         std::vector<uint16_t> ptab;
-        std::vector<long long> poff;
         prime_tables(fac_v, nfac_v, pl->pol_ok, n, ptab, poff);
         if ((rc = dev_upload(A, ptab, &P.prime_tab))) return rc;
         if ((rc = dev_upload(A, poff, &P.prime_off))) return rc;
@@ -1011,6 +1011,34 @@ int build_plan(sht_plan *pl) {
         for (int q = 0; q < kPolicies; ++q) {
             if (!pl->pol_ok[q]) continue;
             if ((rc = dev_upload(A, pl->h_ring_order_v[q], &views[q].ring_order))) return rc;
+            // the rings' records in launch order
+            std::vector<sht::FFTRing> rings(n);
+            for (int r = 0; r < n; ++r) {
+                const int j = pl->h_ring_order_v[q][r];
+                sht::FFTRing &x = rings[r];
+                x = sht::FFTRing{};
+                x.j = j;
+                x.N = g.nlon[j];
+                x.mt = g.mt[j];
+                x.mte = g.mte[j];
+                x.nf = nfac_v[q][j];
+                const int R0 = x.nf > 0 ? fac_v[q][(size_t)j * sht::kMaxFactors] : 0;
+                // an odd first radix of 11 or more is a prime placed first for the MFMA pass
+                // (register passes of 11 and 13 come last; 16 is the only even radix above 8)
+                if (R0 >= sht::kMfmaMinPrime && (R0 & 1)) {
+                    x.prime = R0;
+                    x.M0 = x.N / R0;
+                    x.mulM0 = sht::fastdiv_mul(x.M0);
+                    x.mulR0 = sht::fastdiv_mul(R0);
+                    x.prime_off = poff[R0];
+                }
+                x.tw_off = tw_off[j];
+                x.fboff = g.fboff[j];
+                x.ring_n = g.ring_n[j];
+                x.ring_s = g.ring_s[j];
+                x.passes = (long long)j * sht::kMaxFactors;
+            }
+            if ((rc = dev_upload(A, rings, &views[q].rings))) return rc;
             if ((rc = dev_upload(A, pass_table(g, fac_v[q], nfac_v[q]), &views[q].passes))) return rc;
             if ((rc = dev_upload(A, nfac_v[q], &views[q].nfac))) return rc;
             pl->Pv[q] = views[q];
@@ -1249,9 +1277,10 @@ static StageLaunch plan_stage_launch(const sht_plan *plan, int stage, const Call
             }
             if (stage == sht::kFFTSynthesis)
                 return {sht::fft_kernel(stage, false, c.accumulate != 0), dim3(fft_blocks(plan->g.n, K, c.paired != 0)),
-                        (unsigned)plan->P.fft_smem, nullptr};
+                        (unsigned)plan->P.fft_smem, reinterpret_cast<const Tile *>(plan->P.rings)};
+            // the FFT kernels of a single call take the ring records in place of a tile list
             return {sht::fft_kernel(stage, false), dim3(fft_blocks(plan->g.n, K, false)), (unsigned)plan->P.fft_smem,
-                    nullptr};
+                    reinterpret_cast<const Tile *>(plan->P.rings)};
         }
         default:
             return {sht::legendre_kernel(stage, plan->d.precision, false), dim3(plan->nana[gx][cgs - 1]), 0,
@@ -1970,13 +1999,14 @@ int build_batch(sht_batch *b, const sht_batch_item *items, int n, int flags) {
         const int K = it.K;
         const int cgs = (K + sht::kColsComplex - 1) / sht::kColsComplex;
         const int kgs = (K + pl->P.kg - 1) / pl->P.kg;
-        auto add_fft = [&](int st, int call) {
-            for (int j : pl->h_ring_order_v[pl->pol])
+        auto add_fft = [&](int st, int call) {  // tiles (call, ring rank, column group)
+            const std::vector<int> &order = pl->h_ring_order_v[pl->pol];
+            for (int r = 0; r < (int)order.size(); ++r)
                 for (int q = 0; q < kgs; q += 2) {
                     LaunchUnit u{0.0, {}};
                     for (int h = q; h < std::min(q + 2, kgs); ++h) {
-                        u.tiles.push_back({call, j, h, 0});
-                        u.cost += pl->h_ring_cost_v[pl->pol][j];
+                        u.tiles.push_back({call, r, h, 0});
+                        u.cost += pl->h_ring_cost_v[pl->pol][order[r]];
                     }
                     st_units[st].push_back(u);
                 }
