@@ -130,3 +130,35 @@ def test_uneven_batch_of_64_round_trips(sw, gpu):
         else:  # single-call path, bitwise
             assert torch.equal(out.data, sw.transform(sw.transform(c, S), S).data)
     B.close()
+
+
+@pytest.mark.parametrize("precision", ["fp16x3", "bf16x3"])
+def test_shared_column_groups_mixed_flags(sw, gpu, precision):
+    """Items on one plan share Legendre column groups (column counts that straddle the groups'
+    32 columns), split by the unscale_coslat flag: syntheses with and without it, analyses and
+    a round trip on the same plan, in a split-precision mode, bitwise equal to single calls."""
+    S = plan(sw, 127, nlayers=48, precision=precision)
+    items, expect = [], []
+    for i, (K, unscale) in enumerate([(5, False), (40, True), (7, False), (33, True), (1, False)]):
+        c = spec(sw, S, K, seed=60 + i)
+        f = sw.Field(torch.full((K, S.grid.npoints), float("nan"), device="cuda"), S.grid)
+        items.append((S, c, f, unscale))
+        expect.append(sw.transform(c, S, unscale_coslat=unscale).data)
+    for i, K in enumerate((9, 30)):
+        g = sw.transform(spec(sw, S, K, seed=70 + i), S)
+        out = sw.LowerTriangularArray(torch.full((K, S.spectrum.nharm), float("nan"), dtype=torch.complex64,
+                                                 device="cuda"), S.spectrum)
+        items.append((S, g, out))
+        expect.append(sw.transform(g, S).data)
+    c = spec(sw, S, 11, seed=80)
+    mid = sw.Field(torch.zeros((11, S.grid.npoints), device="cuda"), S.grid)
+    out = sw.LowerTriangularArray.zeros(S.spectrum, 11, device="cuda")
+    items.append((S, c, out, mid))
+    expect.append(sw.transform(sw.transform(c, S), S).data)
+    torch.cuda.synchronize()
+    B = sw.SpectralBatch(items)
+    B.launch()
+    torch.cuda.synchronize()
+    for it, ref in zip(items, expect):
+        assert torch.equal(it[2].data, ref), (it[1].data.shape, ref.shape)
+    B.close()
