@@ -474,6 +474,32 @@ def cpu_baseline(spectrum_o, grid_o, coeffs, seconds=10.0):
             "cpu_model": cpu_model(), "host_cores_available": avail}
 
 
+def copy_bandwidth(torch):
+    """Device copy / read (sum) / write (fill) rates of plain torch kernels over 64 MiB, 256 MiB and
+    1 GiB buffers, HIP events around 50 back-to-back launches: the memory rates a streaming kernel
+    reaches with its data in the Infinity Cache (64 MiB: 128 MiB moved by the copy) and in HBM."""
+    out = {}
+    for mib in (64, 256, 1024):
+        n = mib << 18  # floats
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for name, fn, nbytes in (("copy", lambda: dst.copy_(src), 8 * n), ("read_sum", lambda: src.sum(), 4 * n),
+                                 ("write_fill", lambda: dst.fill_(1.0), 4 * n)):
+            for _ in range(5):
+                fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(50):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / 50
+            out[f"{name}_{mib}MiB"] = {"ms": ms, "bytes": nbytes, "TB_per_s": nbytes / ms / 1e9}
+        del src, dst
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -486,6 +512,8 @@ def main():
     ap.add_argument("--c5-seeds", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run three plain 256 MiB device copies (scale for HBM byte counters)")
+    ap.add_argument("--copy-bandwidth", action="store_true",
+                    help="print the plain device copy / read / write rates as one JSON line and exit")
     ap.add_argument("--tune-cache", default=None,
                     help="autotuner cache file: loaded first if it exists (plans then tune without timing), "
                          "written at the end")
@@ -507,6 +535,10 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_bandwidth:
+        torch.cuda.set_device(0)
+        print(json.dumps({"copy_bandwidth": copy_bandwidth(torch)}), flush=True)
+        return
     import __graft_entry__
     from oracle.sht_oracle import Grid as OGrid, OracleTransform, Spectrum as OSpectrum
     sw = __graft_entry__.load_package()
