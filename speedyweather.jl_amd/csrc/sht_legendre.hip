@@ -200,8 +200,11 @@ __device__ inline void mma_chunk1(const unsigned char *sm, int buf, int p, int r
 //
 // Workgroup = (order m, group of 64 rings, 32 complex columns).  Wave w owns ring tile
 // 4*blk+w (16 rings) and all 64 real columns, for both parities: 8 accumulators.
-// Degrees l = m + 2l' + p stream in chunks of 32 l' per parity; a staging thread loads 8
-// consecutive degrees of one complex column (kk = stage_column(tid), degrees 8(tid%8)..+7).
+// Degrees l = m + 2l' + p stream in chunks of 32 l' per parity; a staging thread loads the
+// degree pairs (2g + 16i, 2g + 16i + 1), i = 0..3, of one complex column (kk = stage_column(tid),
+// g = tid%8), so each load instruction reads 128 contiguous bytes per column, and stages the
+// l' = g + 8i of the chunk at the chunk's K positions 4g + i: the plan stores Λ with the same
+// order of K within every chunk (sht_plan.cpp: syn_kpos).
 
 
 // Per-ring epilogue factors of ring rb < 64 of a synthesis block: 16 bytes in the padding of
@@ -302,7 +305,7 @@ __global__ __launch_bounds__(256, MODE == kBF16x3 ? 2 : 4) void legendre_synthes
     auto load = [&](Ops &S, int c) {
         c = min(c, nch - 1);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) S.v[e] = gload(csrc + min(64 * c + 8 * g + e, L - 1));
+        for (int e = 0; e < 8; ++e) S.v[e] = gload(csrc + min(64 * c + 16 * (e >> 1) + 2 * g + (e & 1), L - 1));
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
             if constexpr (MODE == kFP32) {
@@ -322,14 +325,14 @@ __global__ __launch_bounds__(256, MODE == kBF16x3 ? 2 : 4) void legendre_synthes
             float re[4], im[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const bool ok = kok && 64 * c + 8 * g + 2 * i + p < L;
+                const bool ok = kok && 64 * c + 16 * i + 2 * g + p < L;
                 re[i] = ok ? cur.v[2 * i + p].x : 0.f;
                 im[i] = ok ? cur.v[2 * i + p].y : 0.f;
             }
             if (C.lap) {  // fused Laplacian of the input degrees (uniform)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const float f = C.lap_sign * C.lap_scale * lap_factor(C.lap, m + 64 * c + 8 * g + 2 * i + p);
+                    const float f = C.lap_sign * C.lap_scale * lap_factor(C.lap, m + 64 * c + 16 * i + 2 * g + p);
                     re[i] *= f;
                     im[i] *= f;
                 }

@@ -226,6 +226,11 @@ double fp16_scale(double amax) {
     return std::ldexp(1.0, sh);
 }
 
+// K position within a synthesis chunk of the chunk's l' = ci (ci < 32): the staging thread g
+// of a column holds l' = g + 8i (i = 0..3, degree pairs read as 128-byte runs) and writes them
+// to positions 4g + i, so the Λ fragments take the same order.
+inline int syn_kpos(int ci) { return 4 * (ci % 8) + ci / 8; }
+
 template <class T>
 int dev_upload(std::vector<void *> &allocs, const std::vector<T> &h, const T **out) {
     *out = nullptr;
@@ -748,14 +753,15 @@ int build_plan(sht_plan *pl) {
                 for (int lp = 0; lp < nch * sht::kChunk; ++lp) {
                     const int l = m + 2 * lp + p;
                     if (l >= lmax) continue;
+                    const int c = lp / sht::kChunk, kq = syn_kpos(lp % sht::kChunk);
                     for (int r = 0; r < 16; ++r) {
                         const int j = j0 + 16 * t + r;
                         if (j >= n) continue;
                         const double v = LAM(l, j);
                         if (!split) {
-                            syn_f32[syn_off[m] + ((long long)(p * Jt + t) * (nch * sht::kChunk) + lp) * 16 + r] = (float)v;
+                            syn_f32[syn_off[m] + ((long long)(p * Jt + t) * (nch * sht::kChunk) + c * sht::kChunk + kq) * 16 + r] =
+                                (float)v;
                         } else {
-                            const int c = lp / sht::kChunk, kq = lp % sht::kChunk;
                             const int grp = kq / 8, e = kq % 8, lane = grp * 16 + r;
                             const long long idx = syn_off[m] + (((long long)(p * Jt + t) * nch + c) * 64 + lane) * 8 + e;
                             split_store(v, srow_scale[t * 16 + r], syn_hi, syn_mid, syn_lo, idx);
