@@ -249,7 +249,7 @@ int sht_mlp_destroy(sht_mlp *mlp);
  * small odd radices first or last) and the ring DFT on MFMA (FP32, or FP16 hi/lo on split plans)
  * with 4 or 8 columns per workgroup -- then the launch orders (Legendre tiles grouped per
  * order m on one XCD or plain heaviest-first, each with or without the lightest tiles first;
- * LDS-FFT synthesis column groups paired or not),
+ * LDS-FFT synthesis with all column groups of a ring on one XCD, or in rank order),
  * keeps the fastest in the plan and caches it per (grid, nlat_half, lmax, mmax, nlayers,
  * precision, K) for later plans in the process.  Synchronises the stream; results are
  * unchanged up to the rounding of the Fourier stage's arithmetic. */
@@ -257,7 +257,7 @@ int sht_mlp_destroy(sht_mlp *mlp);
 #define SHT_FOURIER_DFT32 1   /* radix-4 pass + p-point DFT on MFMA, FP32 products */
 #define SHT_FOURIER_DFT16 2   /* the same with scaled FP16 hi/lo products (split plans) */
 int sht_plan_autotune(sht_plan *plan, int32_t K, void *stream);
-/* out[8] = {tuned, MFMA threshold, eights, odd_first, Legendre launch order, FFT pairing,
+/* out[8] = {tuned, MFMA threshold, eights, odd_first, Legendre launch order, FFT grouping,
  * Fourier kernel (SHT_FOURIER_*), DFT columns per workgroup}; the Legendre launch order is
  * bit 0: the tiles of one order m grouped on one XCD, bit 1: the lightest tiles beyond the
  * device's resident workgroups launched first (no tail of late light tiles). */
@@ -269,9 +269,9 @@ int sht_plan_tuning(const sht_plan *plan, int32_t *out);
 int sht_tune_cache_save(const char *path);
 int sht_tune_cache_load(const char *path, int32_t *nloaded);
 /* pins a choice: in[7] = {MFMA threshold (11 or 17), powers of two as 4s / 8s / 16s (eights =
- * 0 / 1 / 2), odd_first, Legendre launch order
- * (0..3, bits as above), FFT pairing, Fourier kernel (SHT_FOURIER_*), DFT columns per workgroup
- * (4 or 8)}.  Every choice's tables are built at plan creation, so this
+ * 0 / 1 / 2), odd_first, Legendre launch order (0..3, bits as above), FFT grouping (1: the
+ * synthesis's column groups of a ring on one XCD), Fourier kernel (SHT_FOURIER_*), DFT columns
+ * per workgroup (4 or 8)}.  Every choice's tables are built at plan creation, so this
  * changes host state only: calls already queued keep what they were launched with.  Not
  * to be called concurrently with other calls on the plan from another host thread. */
 int sht_plan_set_tuning(sht_plan *plan, const int32_t *in);

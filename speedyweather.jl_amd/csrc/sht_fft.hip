@@ -540,11 +540,12 @@ __device__ inline bool fft_tile(const DevPlan &P, const SHT_GLOBAL FFTRing *ring
     } else {
         const int b = blockIdx.x;
         if (paired) {
-            const int npair = (kgs + 1) >> 1;
+            // every column group of ring rank r on XCD r % 8 (blocks go round-robin over the
+            // XCDs), ranks heaviest first on each: the ring's twiddles and tables are fetched
+            // into one L2, and the column groups sharing a line of Fourier rows meet there
             const int s = b / kXCDs;
-            const int u = (s >> 1) * kXCDs + b % kXCDs;
-            r = u / npair;
-            cg = 2 * (u - r * npair) + (s & 1);
+            r = (s / kgs) * kXCDs + b % kXCDs;
+            cg = s - (s / kgs) * kgs;
         } else {
             r = b / kgs;
             cg = b - r * kgs;

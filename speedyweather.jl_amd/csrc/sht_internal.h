@@ -99,7 +99,7 @@ struct CallArgs {
     int plan;             // index into the launch's plan array
     int K;                // columns
     int unscale;          // synthesis: divide by cos(lat)
-    int paired;           // single-call FFT synthesis: paired column-group launch order
+    int paired;           // single-call FFT synthesis: a ring's column groups on one XCD
     const void *in;       // synthesis: coefficients; analysis: grid
     void *out;            // synthesis: grid; analysis: coefficients
     long long ld_in;      // column strides (complex elements for spectra, floats for grids)

@@ -1218,13 +1218,12 @@ int sht_plan_destroy(sht_plan *plan) {
 }
 
 // Blocks of a single-call FFT launch (the kernels map block -> ring, column group):
-// paired, units (ring, pair of column groups) of two blocks round-robin over the XCDs;
-// otherwise one block per (ring, column group).
+// paired, all column groups of a ring on one XCD (ring ranks padded to a multiple of the
+// XCD count); otherwise one block per (ring, column group) in rank order.
 static unsigned fft_blocks(int nrings, int K, bool paired) {
     const int kgs = (K + sht::kFFTCols - 1) / sht::kFFTCols;
     if (!paired) return (unsigned)(nrings * kgs);
-    const long long units = (long long)nrings * ((kgs + 1) / 2);
-    return (unsigned)(2 * sht::kXCDs * ((units + sht::kXCDs - 1) / sht::kXCDs));
+    return (unsigned)(sht::kXCDs * kgs * ((nrings + sht::kXCDs - 1) / sht::kXCDs));
 }
 
 // Launches one stage kernel (every stage kernel has the same signature).

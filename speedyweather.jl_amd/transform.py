@@ -93,7 +93,8 @@ class SpectralTransform:
         odd radices first or last; or the ring
         DFT on MFMA with dft_cols columns per workgroup) and the launch orders: Legendre tiles
         of one order grouped on an XCD, the lightest tiles beyond the resident workgroups
-        launched first, FFT column groups paired (sht_plan_set_tuning)."""
+        launched first, the FFT synthesis's column groups of a ring on one XCD (fft_pairs;
+        sht_plan_set_tuning)."""
         leg = int(bool(legendre_xcd_groups)) | (2 if legendre_light_first else 0)
         v = (C.c_int32 * 7)(int(mfma_min), int(eights), int(bool(odd_first)), leg,
                             int(bool(fft_pairs)), self.FOURIER[fourier], int(dft_cols))
