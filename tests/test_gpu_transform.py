@@ -172,6 +172,37 @@ def test_k_subsets_and_strided_columns(sw, gpu):
     assert rel(back.data.cpu().numpy(), b_ref) < 1e-5
 
 
+@pytest.mark.parametrize("kind, n, lmax, mmax", [("octahedral", 20, 29, 28), ("full", 12, 17, 16)])
+@pytest.mark.parametrize("precision", ["fp32", "fp16x3"])
+def test_ring_counts_not_a_multiple_of_the_xcds(sw, gpu, kind, n, lmax, mmax, precision):
+    """Ring counts that are not multiples of 8: the FFT synthesis puts the column groups of ring
+    rank r on XCD r mod 8 and pads the ranks to a multiple of 8 (the padding blocks exit), with
+    1, 3 and 5 columns (partial column groups); both directions and the round trip against the
+    FP64 oracle, and the same fields with the column groups in rank order (fft_pairs off)."""
+    print(f"ring count case {kind} n={n} {precision}", flush=True)
+    o = oracle(kind, n, lmax, mmax)
+    grid = (sw.OctahedralGaussianGrid if kind == "octahedral" else sw.FullGaussianGrid)(n)
+    sp = sw.Spectrum(lmax, mmax)
+    S = sw.SpectralTransform(sp, grid, nlayers=5, precision=precision)
+    a = sw.seeded_spectrum(sp, 5, seed=31)
+    f_ref = o.synthesis(a)
+    b_ref = o.analysis(f_ref)
+    for K in (1, 3, 5):
+        f = sw.transform(to_dev(sw, a[:K], sp), S)
+        b = sw.transform(f, S)
+        torch.cuda.synchronize()
+        assert rel(f.data.cpu().numpy().reshape(K, -1), f_ref[:K]) < TOL[precision], K
+        assert rel(b.data.cpu().numpy().reshape(K, -1), b_ref[:K]) < TOL[precision], K
+    f_grouped = sw.transform(to_dev(sw, a, sp), S).data.clone()
+    tuning = S.tuning()
+    S.set_tuning(tuning["mfma_min"], tuning["eights"], tuning["odd_first"], tuning["legendre_xcd_groups"],
+                 False, tuning["fourier"], tuning["dft_cols"], tuning["legendre_light_first"])
+    f_plain = sw.transform(to_dev(sw, a, sp), S).data
+    torch.cuda.synchronize()
+    assert torch.equal(f_grouped, f_plain)  # the launch order changes no arithmetic
+    S.close()
+
+
 @pytest.mark.parametrize("offset, extra", [(0, 4), (4, 4), (1, 4), (2, 6)])
 def test_grid_views_aligned_and_unaligned(sw, gpu, offset, extra):
     """Grid views at any float offset and column stride (aligned or not to 8 or 16 bytes)
